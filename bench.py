@@ -1,0 +1,296 @@
+#!/usr/bin/env python3
+"""GST decode benchmark on one MI355X.
+
+Headline workload (printed as one JSON line): 128 seeded textures of uneven size drawn
+uniformly from {512x512, 1024x1024, 512x1024, 2048x2048}, decoded as one batch by the five
+persistent stage kernels replayed from a captured hipGraph.  A "step" is one decode of the
+whole batch with inputs and outputs resident in HBM.  Beside it the same corpus is decoded
+image by image through the plain LoadCompressedDXT path, single 512x512 and 4096x4096
+textures are decoded through the plain path, and the CPU restatement of the decoder
+(oracle/) is timed on the host cores as the CPU baseline.  Every GPU result is checked
+bit-exact against the encoder's blocks (which the oracle reproduces) before it is timed.
+
+Roofline: the dominant kernel's algorithmic bytes per launch (per-stage read+write of
+the decode dataflow) over its average launch time, measured with hipEvents on the
+decode stream, against 8 TB/s.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "decoded Mtexel/s & HBM roofline %, batched vs per-image; p50/p99 latency @QPS"
+HBM_PEAK_GBS = 8000.0
+SIZES_C3 = [(512, 512), (1024, 1024), (512, 1024), (2048, 2048)]
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def corpus_c3(n=128, size_seed=42):
+    rng = np.random.default_rng(size_seed)
+    picks = rng.integers(0, len(SIZES_C3), n)
+    return [(1000 + i, SIZES_C3[p][0], SIZES_C3[p][1]) for i, p in enumerate(picks)]
+
+
+def stage_bytes(hdrs):
+    """Algorithmic read+write bytes per stage for one decode of `hdrs`."""
+    nv = sum(h.num_blocks for h in hdrs)
+    P = sum(h.palette_bytes for h in hdrs)
+    S = sum(h.stream_bytes for h in hdrs)
+    n = len(hdrs)
+    return {
+        "build_table": 2048 * n + 32768 * n,
+        "ans_decode": (S + 32768 * n) + (7 * nv + P),
+        "decode_indices": nv + 4 * nv,
+        "inv_wavelet": 6 * nv + 6 * nv,
+        "assemble": (6 * nv + 4 * nv + P) + 8 * nv,
+    }
+
+
+def pipeline_bytes(hdrs):
+    b_pipe = sum((h.stream_bytes + 2048) + 42 * h.num_blocks + 2 * h.palette_bytes + 65536 for h in hdrs)
+    b_min = sum((h.stream_bytes + 2048) + 8 * h.num_blocks for h in hdrs)
+    return b_pipe, b_min
+
+
+class Gpu:
+    def __init__(self):
+        import torch
+        self.torch = torch
+        self.dev = torch.device("cuda:0")
+
+    @property
+    def stream(self):
+        return self.torch.cuda.current_stream().cuda_stream
+
+    def upload_many(self, blobs, align=256):
+        offs, total = [], 0
+        for b in blobs:
+            offs.append(total)
+            total += (len(b) + align - 1) // align * align
+        host = np.zeros(total, np.uint8)
+        for o, b in zip(offs, blobs):
+            host[o:o + len(b)] = np.frombuffer(b, np.uint8)
+        dev = self.torch.from_numpy(host).to(self.dev)
+        return dev, [dev.data_ptr() + o for o in offs]
+
+    def empty(self, n):
+        return self.torch.empty(int(n), dtype=self.torch.uint8, device=self.dev)
+
+    def timed(self, fn, reps):
+        torch = self.torch
+        start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        start.record()
+        for _ in range(reps):
+            fn()
+        end.record()
+        torch.cuda.synchronize()
+        return start.elapsed_time(end) / reps
+
+
+def check_outputs(gpu, d_out, offsets, dxts, what):
+    gpu.torch.cuda.synchronize()
+    host = d_out.cpu().numpy()
+    for i, (o, d) in enumerate(zip(offsets, dxts)):
+        if host[o:o + len(d)].tobytes() != d:
+            raise SystemExit("%s: image %d differs from the expected DXT1 blocks" % (what, i))
+
+
+def run_batched(gpu, files, dxts, steps, warmup, gpw=0, waves=0, grids=None):
+    import gst_amd
+    hdrs = [gst_amd.ParseHeader(f) for f in files]
+    d_files, file_ptrs = gpu.upload_many(files)
+    out_sizes = [h.dxt_bytes for h in hdrs]
+    out_offs = list(np.cumsum([0] + out_sizes[:-1]))
+    d_out = gpu.empty(sum(out_sizes))
+    b = gst_amd.Batch(len(hdrs), sum(h.num_blocks for h in hdrs), sum(h.palette_bytes for h in hdrs),
+                      groups_per_wave=gpw, waves_per_group=waves, grids=grids)
+    b.bind(hdrs, file_ptrs, [d_out.data_ptr() + int(o) for o in out_offs], gpu.stream)
+    b.decode(gpu.stream)
+    check_outputs(gpu, d_out, out_offs, dxts, "batched eager")
+    b.capture_graph(gpu.stream)
+    d_out.zero_()
+    b.launch_graph(gpu.stream)
+    check_outputs(gpu, d_out, out_offs, dxts, "batched graph")
+    if b.error_bits(gpu.stream):
+        raise SystemExit("batched decode raised device error bits")
+    for _ in range(warmup):
+        b.launch_graph(gpu.stream)
+    ms = gpu.timed(lambda: b.launch_graph(gpu.stream), steps)
+    stage = {k: 0.0 for k in gst_amd.STAGES}
+    reps = 5
+    for _ in range(reps):
+        for k, v in b.decode_timed(gpu.stream).items():
+            stage[k] += v / reps
+    check_outputs(gpu, d_out, out_offs, dxts, "batched timed")
+    dev_bytes = b.device_bytes
+    b.close()
+    del d_files
+    return hdrs, ms, stage, dev_bytes
+
+
+def run_plain(gpu, files, dxts, passes, warmup):
+    """Image by image through LoadCompressedDXT (the reference's per-image drop-in)."""
+    import gst_amd
+    stagings, hdrs = [], []
+    for f in files:
+        s, (h,) = gst_amd.PackStaging([f])
+        stagings.append(s)
+        hdrs.append(h)
+    d_in, in_ptrs = gpu.upload_many(stagings)
+    out_sizes = [h.dxt_bytes for h in hdrs]
+    out_offs = list(np.cumsum([0] + out_sizes[:-1]))
+    d_out = gpu.empty(sum(out_sizes))
+    scratch = sum(gst_amd.RequiredScratchMem(h) for h in hdrs)
+
+    def one_pass():
+        for h, p, o in zip(hdrs, in_ptrs, out_offs):
+            gst_amd.LoadCompressedDXT(h, gpu.stream, p, d_out.data_ptr() + int(o))
+
+    times = []
+    for i in range(warmup + passes):
+        gst_amd.PreallocateDecompressor(scratch)
+        t = gpu.timed(one_pass, 1)
+        gst_amd.FreeDecompressor()
+        if i == 0:
+            check_outputs(gpu, d_out, out_offs, dxts, "plain")
+        if i >= warmup:
+            times.append(t)
+    return hdrs, float(np.median(times))
+
+
+def run_cpu_baseline(files, min_seconds=10.0, max_passes=50):
+    from oracle import gst_oracle as go
+    threads = min(16, os.cpu_count() or 1)
+    texels = 0
+    for f in files:
+        rc, h = go.parse_header(f)
+        texels += h.width * h.height
+    rates = []
+    t_total = 0.0
+    while (t_total < min_seconds or len(rates) < 2) and len(rates) < max_passes:
+        t0 = time.perf_counter()
+        for f in files:
+            rc, _, _ = go.decode_dxt(f, nthreads=threads)
+            assert rc == go.OK
+        dt = time.perf_counter() - t0
+        t_total += dt
+        rates.append(texels / dt / 1e6)
+    return float(np.median(rates)), threads, len(rates), t_total
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--images", type=int, default=128)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-plain", action="store_true", help="skip the per-image comparison")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (--gpus %d requested)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+
+    import gst_amd
+    from gst_amd import encode
+    gpu = Gpu()
+    if not gst_amd.InitializeDecoder(0):
+        raise SystemExit("cuda:0 is not a usable gfx950")
+
+    t0 = time.perf_counter()
+    specs = corpus_c3(args.images)
+    files, dxts = encode.make_textures(specs, nthreads=min(16, os.cpu_count() or 1))
+    log("corpus: %d textures encoded in %.1fs" % (len(files), time.perf_counter() - t0))
+
+    hdrs, ms, stage, dev_bytes = run_batched(gpu, files, dxts, args.steps, args.warmup)
+    texels = sum(h.width * h.height for h in hdrs)
+    value = texels / (ms * 1e-3) / 1e6
+    log("batched: %.3f ms/step  %.0f Mtexel/s  stages %s" % (ms, value, stage))
+
+    sb = stage_bytes(hdrs)
+    dom = max(stage, key=stage.get)
+    achieved = sb[dom] / (stage[dom] * 1e-3) / 1e9
+    b_pipe, b_min = pipeline_bytes(hdrs)
+    result = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "Mtexel/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic",
+        "config": {
+            "workload": "C3 batched: %d seeded textures {512x512,1024x1024,512x1024,2048x2048}, "
+                        "one hipGraph replay of the five persistent stage kernels" % len(hdrs),
+            "images": len(hdrs),
+            "texels": texels,
+            "dxt1_blocks": texels // 16,
+            "compressed_bytes": sum(len(f) for f in files),
+            "plan_device_bytes": dev_bytes,
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom,
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+        },
+        "pipeline_roofline": {
+            "B_pipe_bytes": b_pipe,
+            "B_pipe_frac": round(b_pipe / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4),
+            "B_min_bytes": b_min,
+            "B_min_frac": round(b_min / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4),
+        },
+        "stage_ms": {k: round(v, 4) for k, v in stage.items()},
+        "stage_gbs": {k: round(sb[k] / (stage[k] * 1e-3) / 1e9, 1) for k in stage},
+    }
+
+    if not args.no_plain:
+        _, ms_plain = run_plain(gpu, files, dxts, passes=3, warmup=1)
+        plain = texels / (ms_plain * 1e-3) / 1e6
+        result["per_image"] = {"value": round(plain, 1), "unit": "Mtexel/s", "ms_per_pass": round(ms_plain, 3),
+                               "path": "LoadCompressedDXT per image, 5 launches each"}
+        result["speedup_vs_per_image"] = round(value / plain, 2)
+        log("plain per-image: %.3f ms/pass  %.0f Mtexel/s" % (ms_plain, plain))
+        singles = {}
+        for name, (seed, w, h) in (("C1_512x512", (0, 512, 512)), ("C2_4096x4096", (1, 4096, 4096))):
+            f, d, _ = encode.make_texture(seed, w, h)
+            _, ms_s = run_plain(gpu, [f], [d], passes=5, warmup=2)
+            singles[name] = {"ms": round(ms_s, 4), "Mtexel_s": round(w * h / (ms_s * 1e-3) / 1e6, 1)}
+        result["single_image"] = singles
+        log("single images: %s" % singles)
+
+    if not args.no_cpu:
+        rate, threads, passes, secs = run_cpu_baseline(files)
+        result["cpu_baseline"] = {
+            "value": round(rate, 1), "unit": "Mtexel/s", "cores": threads, "kind": "port",
+            "sample": "the full C3 corpus (%d textures, %.0f Mtexel) decoded %d times in %.1f s by the C "
+                      "restatement, one task per rANS group / wavelet tile / block chunk" %
+                      (len(files), texels / 1e6, passes, secs),
+        }
+        log("cpu baseline: %.0f Mtexel/s on %d threads" % (rate, threads))
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,326 @@
+"""gst_amd: MI355X (gfx950) decoder for GST supercompressed textures.
+
+Python view of libgst_amd.so's C ABI (include/gst_decoder.h), named after the
+reference decoder's entry points (GammaUNC/GST codec/decoder.h:15-37):
+
+    InitializeDecoder      codec/decoder.h:15
+    DecompressDXT          codec/decoder.h:16   host bytes -> host DXT1 bytes
+    LoadCompressedDXT[s]   codec/decoder.h:19,23 device staging buffer -> device output
+    LoadRGB[s]             codec/decoder.h:27,31
+    RequiredScratchMem     codec/decoder.h:35
+    PreallocateDecompressor / FreeDecompressor   codec/decoder.h:36-37
+
+Device buffers and streams are passed as integer addresses / handles (for example
+`tensor.data_ptr()` and `torch.cuda.current_stream().cuda_stream`).  Every call
+raises GstError on a non-zero status.  The library must be built
+(gst_amd/build.py); importing without it raises, there is no fallback path.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgst_amd.so")
+
+GST_OK = 0
+STATUS_NAMES = {
+    -1: "GST_ERR_INVALID_ARGUMENT",
+    -2: "GST_ERR_BAD_HEADER",
+    -3: "GST_ERR_CORRUPT_STREAM",
+    -4: "GST_ERR_OUT_OF_MEMORY",
+    -5: "GST_ERR_HIP",
+    -6: "GST_ERR_NOT_INITIALIZED",
+    -7: "GST_ERR_BUSY",
+}
+DEVERR_STREAM_OFFSETS = 0x1
+DEVERR_FREQUENCIES = 0x2
+DEVERR_PALETTE_INDEX = 0x4
+STAGES = ("build_table", "ans_decode", "decode_indices", "inv_wavelet", "assemble")
+
+
+class GstError(RuntimeError):
+    def __init__(self, code, where):
+        super().__init__("%s failed: %s (%d)" % (where, STATUS_NAMES.get(code, "unknown"), code))
+        self.code = code
+
+
+class Header(ctypes.Structure):
+    """GenTCHeader (codec/codec_base.h:9-20)."""
+    _fields_ = [(name, ctypes.c_uint32) for name in (
+        "width", "height", "palette_bytes", "y_cmp_sz", "chroma_cmp_sz", "palette_sz", "indices_sz")]
+
+    @property
+    def num_blocks(self):
+        return self.width * self.height // 16
+
+    @property
+    def stream_bytes(self):
+        return self.y_cmp_sz + self.chroma_cmp_sz + self.palette_sz + self.indices_sz
+
+    @property
+    def dxt_bytes(self):
+        return self.width * self.height // 2
+
+    def __repr__(self):
+        return "Header(%s)" % ", ".join("%s=%d" % (n, getattr(self, n)) for n, _ in self._fields_)
+
+
+class BatchConfig(ctypes.Structure):
+    _fields_ = [
+        ("capacity_images", ctypes.c_uint32),
+        ("capacity_blocks", ctypes.c_uint64),
+        ("capacity_palette", ctypes.c_uint64),
+        ("ans_groups_per_wave", ctypes.c_uint32),
+        ("ans_waves_per_group", ctypes.c_uint32),
+        ("grid_ans", ctypes.c_uint32),
+        ("grid_wavelet", ctypes.c_uint32),
+        ("grid_indices", ctypes.c_uint32),
+        ("grid_assemble", ctypes.c_uint32),
+        ("rgb_output", ctypes.c_uint32),
+    ]
+
+
+_lib = None
+_P = ctypes.c_void_p
+_U32 = ctypes.c_uint32
+_SZ = ctypes.c_size_t
+_HP = ctypes.POINTER(Header)
+
+
+def _declare(lib):
+    sig = {
+        "gst_initialize_decoder": ([ctypes.c_int], ctypes.c_int),
+        "gst_parse_header": ([ctypes.c_char_p, _SZ, _HP], ctypes.c_int),
+        "gst_decompress_dxt": ([ctypes.c_char_p, _SZ, _P, _SZ], ctypes.c_int),
+        "gst_load_compressed_dxt": ([_HP, _P, _P, _P, _U32, _P, _P], ctypes.c_int),
+        "gst_load_compressed_dxts": ([_HP, _U32, _P, _P, _P, _U32, _P, _P], ctypes.c_int),
+        "gst_load_rgb": ([_HP, _P, _P, _P, _U32, _P, _P], ctypes.c_int),
+        "gst_load_rgbs": ([_HP, _U32, _P, _P, _P, _U32, _P, _P], ctypes.c_int),
+        "gst_required_scratch_mem": ([_HP], _SZ),
+        "gst_preallocate_decompressor": ([_SZ], ctypes.c_int),
+        "gst_free_decompressor": ([], ctypes.c_int),
+        "gst_read_error_bits": ([_P, ctypes.POINTER(_U32)], ctypes.c_int),
+        "gst_staging_size": ([_HP, _U32, ctypes.POINTER(_SZ), ctypes.POINTER(_SZ)], ctypes.c_int),
+        "gst_pack_staging": ([ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(_SZ), _U32, _P, _SZ],
+                             ctypes.c_int),
+        "gst_stage_build_table": ([_P, _P, _P, _P], ctypes.c_int),
+        "gst_stage_ans_decode": ([_HP, _P, _P, _P, _P, _P, _P], ctypes.c_int),
+        "gst_stage_decode_indices": ([_HP, _P, _P, _P, _P], ctypes.c_int),
+        "gst_stage_inv_wavelet": ([_HP, _P, _P, _P], ctypes.c_int),
+        "gst_stage_assemble_dxt": ([_HP, _P, _P, _P, _P, _P, _P], ctypes.c_int),
+        "gst_stage_assemble_rgb": ([_HP, _P, _P, _P, _P, _P, _P], ctypes.c_int),
+        "gst_batch_create": ([ctypes.POINTER(BatchConfig), ctypes.POINTER(_P)], ctypes.c_int),
+        "gst_batch_bind": ([_P, _HP, _U32, ctypes.POINTER(_P), ctypes.POINTER(_P), _P], ctypes.c_int),
+        "gst_batch_decode": ([_P, _P], ctypes.c_int),
+        "gst_batch_capture_graph": ([_P, _P], ctypes.c_int),
+        "gst_batch_launch_graph": ([_P, _P], ctypes.c_int),
+        "gst_batch_decode_timed": ([_P, _P, ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
+        "gst_batch_error_bits": ([_P, _P, ctypes.POINTER(_U32)], ctypes.c_int),
+        "gst_batch_device_bytes": ([_P], _SZ),
+        "gst_batch_destroy": ([_P], ctypes.c_int),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+
+
+def lib():
+    """The loaded libgst_amd.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("libgst_amd.so is not built; run gst_amd/build.py (no fallback exists)")
+        handle = ctypes.CDLL(LIB_PATH)
+        _declare(handle)
+        _lib = handle
+    return _lib
+
+
+def _check(rc, where):
+    if rc != GST_OK:
+        raise GstError(rc, where)
+
+
+def _events(init):
+    init = list(init or ())
+    if not init:
+        return 0, None
+    arr = (_P * len(init))(*init)
+    return len(init), ctypes.cast(arr, _P)
+
+
+# ---- codec/decoder.h mirror -------------------------------------------------------
+
+def InitializeDecoder(device=0):
+    """True when the device is a gfx950 that meets every kernel's requirements."""
+    return lib().gst_initialize_decoder(device) == GST_OK
+
+
+def ParseHeader(data):
+    """GenTCHeader::LoadFrom plus the decoder's size rules (raises GstError)."""
+    hdr = Header()
+    _check(lib().gst_parse_header(bytes(data), len(data), ctypes.byref(hdr)), "ParseHeader")
+    return hdr
+
+
+def DecompressDXT(data):
+    """Host .gtc bytes -> host DXT1 blocks (bytes)."""
+    hdr = ParseHeader(data)
+    out = ctypes.create_string_buffer(hdr.dxt_bytes)
+    _check(lib().gst_decompress_dxt(bytes(data), len(data), out, len(out)), "DecompressDXT")
+    return out.raw
+
+
+def _load(fn, name, hdrs, stream, d_cmp, d_out, init, want_event):
+    hdrs = list(hdrs) if isinstance(hdrs, (list, tuple)) else [hdrs]
+    arr = (Header * len(hdrs))(*hdrs)
+    n_init, init_ptr = _events(init)
+    ev = _P()
+    if name.endswith("s"):
+        rc = fn(arr, len(hdrs), stream, d_cmp, d_out, n_init, init_ptr, ctypes.byref(ev) if want_event else None)
+    else:
+        rc = fn(arr, stream, d_cmp, d_out, n_init, init_ptr, ctypes.byref(ev) if want_event else None)
+    _check(rc, name)
+    return ev.value if want_event else None
+
+
+def LoadCompressedDXT(hdr, stream, d_cmp, d_out, init=(), want_event=False):
+    return _load(lib().gst_load_compressed_dxt, "LoadCompressedDXT", hdr, stream, d_cmp, d_out, init, want_event)
+
+
+def LoadCompressedDXTs(hdrs, stream, d_cmp, d_out, init=(), want_event=False):
+    return _load(lib().gst_load_compressed_dxts, "LoadCompressedDXTs", hdrs, stream, d_cmp, d_out, init,
+                 want_event)
+
+
+def LoadRGB(hdr, stream, d_cmp, d_out, init=(), want_event=False):
+    return _load(lib().gst_load_rgb, "LoadRGB", hdr, stream, d_cmp, d_out, init, want_event)
+
+
+def LoadRGBs(hdrs, stream, d_cmp, d_out, init=(), want_event=False):
+    return _load(lib().gst_load_rgbs, "LoadRGBs", hdrs, stream, d_cmp, d_out, init, want_event)
+
+
+def RequiredScratchMem(hdr):
+    return lib().gst_required_scratch_mem(ctypes.byref(hdr))
+
+
+def PreallocateDecompressor(nbytes):
+    _check(lib().gst_preallocate_decompressor(nbytes), "PreallocateDecompressor")
+
+
+def FreeDecompressor():
+    _check(lib().gst_free_decompressor(), "FreeDecompressor")
+
+
+def ReadErrorBits(stream):
+    bits = _U32()
+    _check(lib().gst_read_error_bits(stream, ctypes.byref(bits)), "ReadErrorBits")
+    return bits.value
+
+
+def StagingSize(hdrs):
+    hdrs = list(hdrs)
+    arr = (Header * len(hdrs))(*hdrs)
+    total, off = _SZ(), _SZ()
+    _check(lib().gst_staging_size(arr, len(hdrs), ctypes.byref(total), ctypes.byref(off)), "StagingSize")
+    return total.value, off.value
+
+
+def PackStaging(files):
+    """Host staging buffer (bytes) for LoadCompressedDXT[s] from .gtc files."""
+    files = [bytes(f) for f in files]
+    hdrs = [ParseHeader(f) for f in files]
+    total, _ = StagingSize(hdrs)
+    out = ctypes.create_string_buffer(total)
+    ptrs = (ctypes.c_char_p * len(files))(*files)
+    lens = (_SZ * len(files))(*[len(f) for f in files])
+    _check(lib().gst_pack_staging(ptrs, lens, len(files), out, total), "PackStaging")
+    return out.raw, hdrs
+
+
+# ---- one launch per stage ------------------------------------------------------------
+
+def StageBuildTable(d_freqs, d_tables, d_err, stream):
+    _check(lib().gst_stage_build_table(d_freqs, d_tables, d_err, stream), "StageBuildTable")
+
+
+def StageAnsDecode(hdr, d_streams, d_tables, d_symbols, d_group_sums, d_err, stream):
+    _check(lib().gst_stage_ans_decode(ctypes.byref(hdr), d_streams, d_tables, d_symbols, d_group_sums,
+                                      d_err, stream), "StageAnsDecode")
+
+
+def StageDecodeIndices(hdr, d_symbols, d_group_sums, d_indices, stream):
+    _check(lib().gst_stage_decode_indices(ctypes.byref(hdr), d_symbols, d_group_sums, d_indices, stream),
+           "StageDecodeIndices")
+
+
+def StageInvWavelet(hdr, d_symbols, d_planes, stream):
+    _check(lib().gst_stage_inv_wavelet(ctypes.byref(hdr), d_symbols, d_planes, stream), "StageInvWavelet")
+
+
+def StageAssembleDXT(hdr, d_symbols, d_planes, d_indices, d_out, d_err, stream):
+    _check(lib().gst_stage_assemble_dxt(ctypes.byref(hdr), d_symbols, d_planes, d_indices, d_out, d_err,
+                                        stream), "StageAssembleDXT")
+
+
+def StageAssembleRGB(hdr, d_symbols, d_planes, d_indices, d_out, d_err, stream):
+    _check(lib().gst_stage_assemble_rgb(ctypes.byref(hdr), d_symbols, d_planes, d_indices, d_out, d_err,
+                                        stream), "StageAssembleRGB")
+
+
+# ---- batched decode of uneven sizes ------------------------------------------------
+
+class Batch:
+    """A device-resident plan for up to `capacity_images` textures of any legal size."""
+
+    def __init__(self, capacity_images, capacity_blocks, capacity_palette=0, groups_per_wave=0,
+                 waves_per_group=0, grids=None, rgb=False):
+        grids = grids or {}
+        cfg = BatchConfig(capacity_images, capacity_blocks, capacity_palette, groups_per_wave,
+                          waves_per_group, grids.get("ans_decode", 0), grids.get("inv_wavelet", 0),
+                          grids.get("decode_indices", 0), grids.get("assemble", 0), 1 if rgb else 0)
+        self._h = _P()
+        _check(lib().gst_batch_create(ctypes.byref(cfg), ctypes.byref(self._h)), "gst_batch_create")
+
+    def bind(self, hdrs, d_files, d_outs, stream):
+        hdrs = list(hdrs)
+        n = len(hdrs)
+        arr = (Header * n)(*hdrs)
+        files = (_P * n)(*d_files)
+        outs = (_P * n)(*d_outs)
+        _check(lib().gst_batch_bind(self._h, arr, n, files, outs, stream), "gst_batch_bind")
+
+    def decode(self, stream):
+        _check(lib().gst_batch_decode(self._h, stream), "gst_batch_decode")
+
+    def capture_graph(self, stream):
+        _check(lib().gst_batch_capture_graph(self._h, stream), "gst_batch_capture_graph")
+
+    def launch_graph(self, stream):
+        _check(lib().gst_batch_launch_graph(self._h, stream), "gst_batch_launch_graph")
+
+    def decode_timed(self, stream):
+        ms = (ctypes.c_float * 5)()
+        _check(lib().gst_batch_decode_timed(self._h, stream, ms), "gst_batch_decode_timed")
+        return dict(zip(STAGES, list(ms)))
+
+    def error_bits(self, stream):
+        bits = _U32()
+        _check(lib().gst_batch_error_bits(self._h, stream, ctypes.byref(bits)), "gst_batch_error_bits")
+        return bits.value
+
+    @property
+    def device_bytes(self):
+        return lib().gst_batch_device_bytes(self._h)
+
+    def close(self):
+        if self._h:
+            lib().gst_batch_destroy(self._h)
+            self._h = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

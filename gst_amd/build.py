@@ -1,0 +1,82 @@
+"""Builds the native libraries in-tree.
+
+  gst_amd/libgst_amd.so     the gfx950 decoder (HIP kernels + host orchestration + C ABI)
+  gst_amd/libgst_encode.so  CPU texture generator and GST encoder (input tooling)
+  oracle/liboracle_gst.so   the CPU restatement used as the checker (test-only)
+  oracle/_ref/ref_driver    the reference's encoder and CPU stage functions, only where
+                            the reference sources are present (test-only)
+
+Every call compiles; outputs are written to a temporary name and renamed, so a
+reader never maps a half-written library.
+"""
+import os
+import shutil
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+INCLUDE = os.path.join(ROOT, "include")
+REFERENCE = "/root/reference"
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+HIP_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "-Wall", "-Wno-unused-command-line-argument",
+    # No packed FP32 instructions anywhere in the device code.
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+
+
+def _run(cmd, cwd=None):
+    proc = subprocess.run(cmd, cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if proc.returncode != 0:
+        raise RuntimeError("build failed (%s):\n%s" % (" ".join(cmd), proc.stdout))
+    return proc.stdout
+
+
+def _atomic_build(out, cmd_for):
+    tmp = out + ".tmp%d" % os.getpid()
+    try:
+        _run(cmd_for(tmp))
+        os.replace(tmp, out)
+    finally:
+        if os.path.exists(tmp):
+            os.unlink(tmp)
+    return out
+
+
+def build_decoder():
+    out = os.path.join(HERE, "libgst_amd.so")
+    srcs = [os.path.join(CSRC, "gst_kernels.hip"), os.path.join(CSRC, "gst_decoder.cpp"),
+            os.path.join(CSRC, "gst_serve.cpp")]
+    srcs = [s for s in srcs if os.path.exists(s)]
+    return _atomic_build(out, lambda o: [HIPCC, *HIP_FLAGS, "-I" + INCLUDE, "-I" + CSRC, *srcs,
+                                         "-o", o, "-lpthread"])
+
+
+def build_encoder():
+    out = os.path.join(HERE, "libgst_encode.so")
+    cxx = shutil.which("g++") or "g++"
+    return _atomic_build(out, lambda o: [cxx, "-std=c++17", "-O2", "-Wall", "-Wextra", "-fPIC", "-shared",
+                                         "-pthread", "-I" + INCLUDE, os.path.join(CSRC, "gst_encode.cpp"),
+                                         "-o", o])
+
+
+def build_oracle():
+    """The checker libraries (not part of the product)."""
+    oracle = os.path.join(ROOT, "oracle")
+    _run(["make", "-s", "-B", "-C", oracle, os.path.join(oracle, "liboracle_gst.so")])
+    if os.path.isdir(REFERENCE):
+        _run(["make", "-s", "-C", oracle, "ref"])
+
+
+def build_all():
+    build_decoder()
+    build_encoder()
+    build_oracle()
+
+
+if __name__ == "__main__":
+    build_all()
+    print("built:", os.path.join(HERE, "libgst_amd.so"), os.path.join(HERE, "libgst_encode.so"))

@@ -1,0 +1,863 @@
+// gst_decoder.cpp -- host orchestration and the C ABI of libgst_amd.so.
+//
+// Mirrors the reference decoder's host side (codec/decoder.cpp): header checks,
+// offset computation, scratch carving, and the five-stage launch sequence
+// build_table -> ans_decode -> decode_indices -> inv_wavelet -> assemble on one
+// stream (stream order replaces the reference's OpenCL event DAG, :98-428).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "gst_decoder.h"
+#include "gst_internal.h"
+
+using gst::ImageDesc;
+using gst::KernelArgs;
+using gst::PlanHeader;
+using gst::StageConfig;
+
+namespace {
+
+constexpr size_t kAlign = 256;
+size_t AlignUp(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+struct Globals {
+  std::mutex mu;
+  bool initialized = false;
+  int device = 0;
+  int cu_count = 0;
+  // Preallocated scratch (PreallocateDecompressor): bump allocation in call order.
+  uint8_t *pre_base = nullptr;
+  size_t pre_size = 0;
+  size_t pre_off = 0;
+  // Accumulated device error bits per stream.
+  std::map<hipStream_t, uint32_t *> err_words;
+};
+
+Globals &G() {
+  static Globals g;
+  return g;
+}
+
+int HipFail(hipError_t e) {
+  (void)hipGetLastError();
+  return e == hipErrorOutOfMemory ? GST_ERR_OUT_OF_MEMORY : GST_ERR_HIP;
+}
+
+#define GST_HIP(call)                         \
+  do {                                        \
+    hipError_t _e = (call);                   \
+    if (_e != hipSuccess) return HipFail(_e); \
+  } while (0)
+
+int EnsureInit() {
+  Globals &g = G();
+  if (g.initialized) return GST_OK;
+  return gst_initialize_decoder(0) == GST_OK ? GST_OK : GST_ERR_NOT_INITIALIZED;
+}
+
+// Header rules: README.md:67-71 (w, h multiples of 128; w*h multiple of 16*256*32),
+// codec/encoder.cpp:100-108 (palette padded to 8192 B), codec/entropy.cpp:202-262
+// (stream = u32 end offsets + groups of >= 128 bytes, multiple of 4 bytes).
+int CheckHeader(const gst_header &h) {
+  const uint64_t w = h.width, hh = h.height;
+  if (w == 0 || hh == 0 || (w % 128) || (hh % 128) || ((w * hh) % 131072)) return GST_ERR_BAD_HEADER;
+  const uint64_t nv = w * hh / 16;
+  if (nv > (1ull << 24)) return GST_ERR_BAD_HEADER;
+  const uint64_t P = h.palette_bytes;
+  if (P == 0 || (P % gst::kGroupSyms) || P > 4 * nv) return GST_ERR_BAD_HEADER;
+  const uint64_t syms[4] = {2 * nv, 4 * nv, P, nv};
+  const uint64_t sz[4] = {h.y_cmp_sz, h.chroma_cmp_sz, h.palette_sz, h.indices_sz};
+  uint64_t total = 0;
+  for (int k = 0; k < 4; ++k) {
+    if (sz[k] % 4) return GST_ERR_BAD_HEADER;
+    if (sz[k] < syms[k] / gst::kGroupSyms * (4 + 4 * gst::kLanesPerGroup)) return GST_ERR_BAD_HEADER;
+    total += sz[k];
+  }
+  if (total >= (1ull << 31)) return GST_ERR_BAD_HEADER;
+  return GST_OK;
+}
+
+uint64_t StreamBytes(const gst_header &h) {
+  return static_cast<uint64_t>(h.y_cmp_sz) + h.chroma_cmp_sz + h.palette_sz + h.indices_sz;
+}
+
+uint32_t NumBlocks(const gst_header &h) { return h.width / 4 * (h.height / 4); }
+
+// Per-image scratch carve-up (the call's tables and status words sit before it).
+struct Carve {
+  size_t symbols, planes, indices, group_sums, total;
+};
+
+Carve CarveFor(const gst_header &h) {
+  const size_t nv = NumBlocks(h), P = h.palette_bytes;
+  Carve c;
+  size_t off = 0;
+  c.symbols = off; off += AlignUp(7 * nv + P, kAlign);
+  c.planes = off; off += AlignUp(6 * nv, kAlign);
+  c.indices = off; off += AlignUp(4 * nv, kAlign);
+  c.group_sums = off; off += AlignUp(4 * (nv / gst::kGroupSyms), kAlign);
+  c.total = off;
+  return c;
+}
+
+void FillDesc(const gst_header &h, const uint8_t *freqs, const uint8_t *streams, uint8_t *scratch,
+              const Carve &c, uint32_t table_base, uint8_t *out, bool rgb, ImageDesc *d) {
+  const uint32_t nv = NumBlocks(h);
+  memset(d, 0, sizeof(*d));
+  d->width = h.width;
+  d->height = h.height;
+  d->nv = nv;
+  d->bx = h.width / 4;
+  d->palette_bytes = h.palette_bytes;
+  const uint32_t syms[4] = {2 * nv, 4 * nv, h.palette_bytes, nv};
+  const uint32_t sz[4] = {h.y_cmp_sz, h.chroma_cmp_sz, h.palette_sz, h.indices_sz};
+  size_t in_off = 0;
+  for (int k = 0; k < 4; ++k) {
+    d->groups[k] = syms[k] / gst::kGroupSyms;
+    d->stream_len[k] = sz[k];
+    d->streams[k] = streams + in_off;
+    in_off += sz[k];
+  }
+  d->table_base = table_base;
+  d->rgb = rgb ? 1 : 0;
+  d->freqs = freqs;
+  d->symbols = scratch + c.symbols;
+  d->planes = reinterpret_cast<int8_t *>(scratch + c.planes);
+  d->indices = reinterpret_cast<int32_t *>(scratch + c.indices);
+  d->group_sums = reinterpret_cast<int32_t *>(scratch + c.group_sums);
+  d->out = out;
+}
+
+StageConfig DefaultStageConfig(bool rgb) {
+  StageConfig s;
+  s.ans_waves = 4;
+  s.ans_groups_per_wave = 2;
+  s.rgb = rgb ? 1 : 0;
+  return s;
+}
+
+uint32_t GroupsPerUnit(const StageConfig &s) { return s.ans_waves * s.ans_groups_per_wave; }
+
+uint32_t *ErrWord(hipStream_t s) {
+  Globals &g = G();
+  std::lock_guard<std::mutex> lk(g.mu);
+  auto it = g.err_words.find(s);
+  if (it != g.err_words.end()) return it->second;
+  uint32_t *w = nullptr;
+  if (hipMalloc(&w, sizeof(uint32_t)) != hipSuccess) return nullptr;
+  if (hipMemset(w, 0, sizeof(uint32_t)) != hipSuccess) {
+    (void)hipFree(w);
+    return nullptr;
+  }
+  g.err_words[s] = w;
+  return w;
+}
+
+// Pinned host staging for small descriptor uploads: slots recycled once the copy
+// that last used them has completed.
+class PinnedRing {
+ public:
+  static PinnedRing &Get() {
+    static PinnedRing r;
+    return r;
+  }
+  int Upload(void *d_dst, const void *src, size_t n, hipStream_t s) {
+    std::lock_guard<std::mutex> lk(mu_);
+    if (slots_.empty()) {
+      for (int i = 0; i < kSlots; ++i) {
+        Slot sl;
+        GST_HIP(hipHostMalloc(&sl.host, kSlotBytes, hipHostMallocDefault));
+        GST_HIP(hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming));
+        sl.used = false;
+        slots_.push_back(sl);
+      }
+    }
+    if (n > kSlotBytes) {
+      void *h = nullptr;
+      GST_HIP(hipHostMalloc(&h, n, hipHostMallocDefault));
+      memcpy(h, src, n);
+      hipError_t e = hipMemcpyAsync(d_dst, h, n, hipMemcpyHostToDevice, s);
+      if (e == hipSuccess) e = hipStreamSynchronize(s);
+      (void)hipHostFree(h);
+      GST_HIP(e);
+      return GST_OK;
+    }
+    Slot &sl = slots_[next_];
+    next_ = (next_ + 1) % kSlots;
+    if (sl.used) GST_HIP(hipEventSynchronize(sl.ev));
+    memcpy(sl.host, src, n);
+    GST_HIP(hipMemcpyAsync(d_dst, sl.host, n, hipMemcpyHostToDevice, s));
+    GST_HIP(hipEventRecord(sl.ev, s));
+    sl.used = true;
+    return GST_OK;
+  }
+
+ private:
+  static constexpr int kSlots = 16;
+  static constexpr size_t kSlotBytes = 64 << 10;
+  struct Slot {
+    void *host;
+    hipEvent_t ev;
+    bool used;
+  };
+  std::mutex mu_;
+  std::vector<Slot> slots_;
+  int next_ = 0;
+};
+
+// Plan blob: PlanHeader | ImageDesc[cap] | prefix[5][cap + 1].
+size_t PlanBytes(uint32_t cap) {
+  return AlignUp(sizeof(PlanHeader), 64) + AlignUp(sizeof(ImageDesc) * cap, 64) +
+         AlignUp(sizeof(uint32_t) * gst::kNumStages * (cap + 1), 64);
+}
+
+void BuildPlan(const std::vector<ImageDesc> &descs, uint32_t cap, uint32_t groups_per_unit,
+               std::vector<uint8_t> *blob, uint32_t units_out[gst::kNumStages]) {
+  blob->assign(PlanBytes(cap), 0);
+  PlanHeader *ph = reinterpret_cast<PlanHeader *>(blob->data());
+  ImageDesc *im = reinterpret_cast<ImageDesc *>(blob->data() + AlignUp(sizeof(PlanHeader), 64));
+  uint32_t *pre = reinterpret_cast<uint32_t *>(blob->data() + AlignUp(sizeof(PlanHeader), 64) +
+                                               AlignUp(sizeof(ImageDesc) * cap, 64));
+  const uint32_t n = static_cast<uint32_t>(descs.size());
+  ph->n_images = n;
+  ph->capacity = cap;
+  ph->groups_per_unit = groups_per_unit;
+  for (uint32_t st = 0; st < gst::kNumStages; ++st) {
+    uint32_t acc = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+      pre[st * (cap + 1) + i] = acc;
+      acc += gst::StageUnitsOf(descs[i], st, groups_per_unit);
+    }
+    for (uint32_t i = n; i <= cap; ++i) pre[st * (cap + 1) + i] = acc;
+    ph->units[st] = acc;
+    units_out[st] = acc;
+  }
+  memcpy(im, descs.data(), sizeof(ImageDesc) * n);
+}
+
+KernelArgs PlanArgs(uint8_t *d_blob, uint32_t cap, uint32_t *err, uint32_t *tables, uint32_t *status) {
+  KernelArgs a;
+  memset(&a, 0, sizeof(a));
+  a.plan = reinterpret_cast<const PlanHeader *>(d_blob);
+  a.images = reinterpret_cast<const ImageDesc *>(d_blob + AlignUp(sizeof(PlanHeader), 64));
+  a.prefix = reinterpret_cast<const uint32_t *>(d_blob + AlignUp(sizeof(PlanHeader), 64) +
+                                                AlignUp(sizeof(ImageDesc) * cap, 64));
+  a.err = err;
+  a.tables = tables;
+  a.table_status = status;
+  return a;
+}
+
+int LaunchAll(const KernelArgs &a, const uint32_t grids[gst::kNumStages], const StageConfig &cfg,
+              hipStream_t s) {
+  for (uint32_t st = 0; st < gst::kNumStages; ++st)
+    GST_HIP(gst::LaunchStage(st, a, grids[st], cfg, s));
+  return GST_OK;
+}
+
+// Scratch for one decode call: the preallocation when it has room, otherwise
+// stream-ordered device memory released on the stream after the last kernel.
+struct Scratch {
+  uint8_t *ptr = nullptr;
+  bool async = false;
+};
+
+int TakeScratch(size_t bytes, hipStream_t s, Scratch *out) {
+  Globals &g = G();
+  {
+    std::lock_guard<std::mutex> lk(g.mu);
+    if (g.pre_base) {
+      const size_t off = AlignUp(g.pre_off, kAlign);
+      if (off + bytes <= g.pre_size) {
+        out->ptr = g.pre_base + off;
+        out->async = false;
+        g.pre_off = off + bytes;
+        return GST_OK;
+      }
+    }
+  }
+  void *p = nullptr;
+  GST_HIP(hipMallocAsync(&p, bytes, s));
+  out->ptr = static_cast<uint8_t *>(p);
+  out->async = true;
+  return GST_OK;
+}
+
+// Scratch of one decode call: [tables 4n x 8 KiB][status 4n u32][plan blob if n > 1]
+// [per-image carves].
+struct CallLayout {
+  size_t tables, status, plan, images, total;
+  std::vector<size_t> carve_off;
+  std::vector<Carve> carves;
+};
+
+CallLayout LayoutCall(const gst_header *hdrs, uint32_t n) {
+  CallLayout L;
+  size_t off = 0;
+  L.tables = off; off += AlignUp(4ull * n * gst::kTableSize * 4, kAlign);
+  L.status = off; off += AlignUp(4ull * n * 4, kAlign);
+  L.plan = off; off += n > 1 ? AlignUp(PlanBytes(n), kAlign) : 0;
+  L.images = off;
+  L.carves.resize(n);
+  L.carve_off.resize(n);
+  for (uint32_t i = 0; i < n; ++i) {
+    L.carves[i] = CarveFor(hdrs[i]);
+    L.carve_off[i] = off;
+    off += L.carves[i].total;
+  }
+  L.total = off;
+  return L;
+}
+
+// n images from a staging buffer in the reference layout (codec/decoder.cpp:98-428).
+int DecodeStaging(const gst_header *hdrs, uint32_t n, hipStream_t s, const void *d_cmp, void *d_out,
+                  bool rgb, uint32_t num_init, const gst_event *init, gst_event *done) {
+  if (!hdrs || n == 0 || !d_cmp || !d_out || (num_init && !init)) return GST_ERR_INVALID_ARGUMENT;
+  int rc = EnsureInit();
+  if (rc != GST_OK) return rc;
+  for (uint32_t i = 0; i < n; ++i)
+    if ((rc = CheckHeader(hdrs[i])) != GST_OK) return rc;
+  uint32_t *err = ErrWord(s);
+  if (!err) return GST_ERR_HIP;
+  for (uint32_t i = 0; i < num_init; ++i)
+    GST_HIP(hipStreamWaitEvent(s, static_cast<hipEvent_t>(init[i]), 0));
+
+  // codec/decoder.cpp:109-111,172-209: offsets block, n x 2048 B frequencies, streams.
+  const size_t offsets_bytes = AlignUp(32ull * n, 512);
+  const uint8_t *base = static_cast<const uint8_t *>(d_cmp);
+  const uint8_t *streams = base + offsets_bytes + 2048ull * n;
+  const CallLayout L = LayoutCall(hdrs, n);
+  Scratch sc;
+  if ((rc = TakeScratch(L.total, s, &sc)) != GST_OK) return rc;
+
+  const StageConfig cfg = DefaultStageConfig(rgb);
+  std::vector<ImageDesc> descs(n);
+  size_t in_off = 0, out_off = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    FillDesc(hdrs[i], base + offsets_bytes + 2048ull * i, streams + in_off, sc.ptr + L.carve_off[i],
+             L.carves[i], 4 * i, static_cast<uint8_t *>(d_out) + out_off, rgb, &descs[i]);
+    in_off += StreamBytes(hdrs[i]);
+    out_off += rgb ? 3ull * hdrs[i].width * hdrs[i].height : hdrs[i].width * hdrs[i].height / 2ull;
+  }
+  uint32_t *tables = reinterpret_cast<uint32_t *>(sc.ptr + L.tables);
+  uint32_t *status = reinterpret_cast<uint32_t *>(sc.ptr + L.status);
+  uint32_t grids[gst::kNumStages];
+  if (n == 1) {
+    KernelArgs a;
+    memset(&a, 0, sizeof(a));
+    a.single = descs[0];
+    a.err = err;
+    a.tables = tables;
+    a.table_status = status;
+    for (uint32_t st = 0; st < gst::kNumStages; ++st)
+      grids[st] = gst::StageUnitsOf(descs[0], st, GroupsPerUnit(cfg));
+    rc = LaunchAll(a, grids, cfg, s);
+  } else {
+    std::vector<uint8_t> blob;
+    uint32_t units[gst::kNumStages];
+    BuildPlan(descs, n, GroupsPerUnit(cfg), &blob, units);
+    uint8_t *d_blob = sc.ptr + L.plan;
+    rc = PinnedRing::Get().Upload(d_blob, blob.data(), blob.size(), s);
+    if (rc == GST_OK) {
+      const KernelArgs a = PlanArgs(d_blob, n, err, tables, status);
+      rc = LaunchAll(a, units, cfg, s);
+    }
+  }
+  if (sc.async) {
+    const hipError_t e = hipFreeAsync(sc.ptr, s);
+    if (rc == GST_OK && e != hipSuccess) rc = HipFail(e);
+  }
+  if (rc != GST_OK) return rc;
+  if (done) {
+    hipEvent_t ev;
+    GST_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    GST_HIP(hipEventRecord(ev, s));
+    *done = ev;
+  }
+  return GST_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+
+extern "C" {
+
+int gst_initialize_decoder(int device) {
+  Globals &g = G();
+  std::lock_guard<std::mutex> lk(g.mu);
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) {
+    (void)hipGetLastError();
+    return GST_ERR_NOT_INITIALIZED;
+  }
+  GST_HIP(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  GST_HIP(hipGetDeviceProperties(&prop, device));
+  if (std::string(prop.gcnArchName).rfind("gfx950", 0) != 0) return GST_ERR_NOT_INITIALIZED;
+  // The kernels need 256-thread work-groups and 16 KiB + 8.4 KiB of LDS
+  // (codec/decoder.cpp:558-591 checks the work-group sizes of the reference's).
+  if (prop.maxThreadsPerBlock < 512 || prop.sharedMemPerBlock < 32768) return GST_ERR_NOT_INITIALIZED;
+  g.device = device;
+  g.cu_count = prop.multiProcessorCount;
+  g.initialized = true;
+  return GST_OK;
+}
+
+int gst_parse_header(const uint8_t *file, size_t len, gst_header *hdr) {
+  if (!file || !hdr) return GST_ERR_INVALID_ARGUMENT;
+  if (len < sizeof(gst_header) + 2048) return GST_ERR_BAD_HEADER;
+  memcpy(hdr, file, sizeof(gst_header));
+  const int rc = CheckHeader(*hdr);
+  if (rc != GST_OK) return rc;
+  if (sizeof(gst_header) + 2048 + StreamBytes(*hdr) > len) return GST_ERR_BAD_HEADER;
+  return GST_OK;
+}
+
+size_t gst_required_scratch_mem(const gst_header *hdr) {
+  if (!hdr) return 0;
+  // codec/decoder.cpp:41-47
+  return 4 * 2048 * 6 + 17ull * hdr->width * hdr->height / 16 + hdr->palette_bytes;
+}
+
+int gst_preallocate_decompressor(size_t bytes) {
+  int rc = EnsureInit();
+  if (rc != GST_OK) return rc;
+  Globals &g = G();
+  std::lock_guard<std::mutex> lk(g.mu);
+  if (g.pre_base) {
+    GST_HIP(hipDeviceSynchronize());
+    GST_HIP(hipFree(g.pre_base));
+    g.pre_base = nullptr;
+  }
+  void *p = nullptr;
+  GST_HIP(hipMalloc(&p, bytes));
+  g.pre_base = static_cast<uint8_t *>(p);
+  g.pre_size = bytes;
+  g.pre_off = 0;
+  return GST_OK;
+}
+
+int gst_free_decompressor(void) {
+  Globals &g = G();
+  std::lock_guard<std::mutex> lk(g.mu);
+  if (g.pre_base) {
+    GST_HIP(hipDeviceSynchronize());
+    GST_HIP(hipFree(g.pre_base));
+  }
+  g.pre_base = nullptr;
+  g.pre_size = g.pre_off = 0;
+  return GST_OK;
+}
+
+int gst_read_error_bits(gst_stream stream, uint32_t *bits) {
+  if (!bits) return GST_ERR_INVALID_ARGUMENT;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  uint32_t *w = ErrWord(s);
+  if (!w) return GST_ERR_HIP;
+  GST_HIP(hipStreamSynchronize(s));
+  GST_HIP(hipMemcpy(bits, w, sizeof(uint32_t), hipMemcpyDeviceToHost));
+  GST_HIP(hipMemset(w, 0, sizeof(uint32_t)));
+  return GST_OK;
+}
+
+int gst_staging_size(const gst_header *hdrs, uint32_t n, size_t *total, size_t *streams_offset) {
+  if (!hdrs || n == 0 || !total) return GST_ERR_INVALID_ARGUMENT;
+  size_t bytes = AlignUp(32ull * n, 512) + 2048ull * n;
+  if (streams_offset) *streams_offset = bytes;
+  for (uint32_t i = 0; i < n; ++i) bytes += StreamBytes(hdrs[i]);
+  *total = bytes;
+  return GST_OK;
+}
+
+int gst_pack_staging(const uint8_t *const *files, const size_t *lens, uint32_t n, uint8_t *out,
+                     size_t cap) {
+  if (!files || !lens || n == 0 || !out) return GST_ERR_INVALID_ARGUMENT;
+  std::vector<gst_header> hdrs(n);
+  for (uint32_t i = 0; i < n; ++i) {
+    const int rc = gst_parse_header(files[i], lens[i], &hdrs[i]);
+    if (rc != GST_OK) return rc;
+  }
+  size_t total = 0, soff = 0;
+  gst_staging_size(hdrs.data(), n, &total, &soff);
+  if (total > cap) return GST_ERR_INVALID_ARGUMENT;
+  const size_t offsets_bytes = AlignUp(32ull * n, 512);
+  memset(out, 0, offsets_bytes);
+  uint32_t *out_off = reinterpret_cast<uint32_t *>(out);
+  uint32_t *in_off = out_off + 4 * n;
+  uint32_t oo = 0, io = 0;
+  uint8_t *data = out + soff;
+  for (uint32_t i = 0; i < n; ++i) {
+    const gst_header &h = hdrs[i];
+    const uint32_t nv = NumBlocks(h);
+    // demo/photos_sf.cpp:778-789
+    out_off[4 * i + 0] = oo; oo += 2 * nv;
+    out_off[4 * i + 1] = oo; oo += 4 * nv;
+    out_off[4 * i + 2] = oo; oo += h.palette_bytes;
+    out_off[4 * i + 3] = oo; oo += nv;
+    in_off[4 * i + 0] = io; io += h.y_cmp_sz;
+    in_off[4 * i + 1] = io; io += h.chroma_cmp_sz;
+    in_off[4 * i + 2] = io; io += h.palette_sz;
+    in_off[4 * i + 3] = io; io += h.indices_sz;
+    memcpy(out + offsets_bytes + 2048ull * i, files[i] + sizeof(gst_header), 2048);
+    const size_t sb = StreamBytes(h);
+    memcpy(data, files[i] + sizeof(gst_header) + 2048, sb);
+    data += sb;
+  }
+  return GST_OK;
+}
+
+int gst_load_compressed_dxt(const gst_header *hdr, gst_stream stream, const void *d_cmp, void *d_out,
+                            uint32_t num_init, const gst_event *init, gst_event *done) {
+  return DecodeStaging(hdr, 1, static_cast<hipStream_t>(stream), d_cmp, d_out, false, num_init, init,
+                       done);
+}
+
+int gst_load_compressed_dxts(const gst_header *hdrs, uint32_t n, gst_stream stream, const void *d_cmp,
+                             void *d_out, uint32_t num_init, const gst_event *init, gst_event *done) {
+  return DecodeStaging(hdrs, n, static_cast<hipStream_t>(stream), d_cmp, d_out, false, num_init, init,
+                       done);
+}
+
+int gst_load_rgb(const gst_header *hdr, gst_stream stream, const void *d_cmp, void *d_out,
+                 uint32_t num_init, const gst_event *init, gst_event *done) {
+  return DecodeStaging(hdr, 1, static_cast<hipStream_t>(stream), d_cmp, d_out, true, num_init, init,
+                       done);
+}
+
+int gst_load_rgbs(const gst_header *hdrs, uint32_t n, gst_stream stream, const void *d_cmp,
+                  void *d_out, uint32_t num_init, const gst_event *init, gst_event *done) {
+  return DecodeStaging(hdrs, n, static_cast<hipStream_t>(stream), d_cmp, d_out, true, num_init, init,
+                       done);
+}
+
+int gst_decompress_dxt(const uint8_t *file, size_t len, uint8_t *out, size_t out_len) {
+  if (!file || !out) return GST_ERR_INVALID_ARGUMENT;
+  gst_header h;
+  int rc = gst_parse_header(file, len, &h);
+  if (rc != GST_OK) return rc;
+  const size_t dxt = static_cast<size_t>(h.width) * h.height / 2;
+  if (out_len < dxt) return GST_ERR_INVALID_ARGUMENT;
+  if ((rc = EnsureInit()) != GST_OK) return rc;
+  // codec/decoder.cpp:430-476: offsets block of 512 B, then the file minus its header.
+  const size_t body = sizeof(gst_header) + 2048 + StreamBytes(h) - sizeof(gst_header);
+  std::vector<uint8_t> staging(512 + body, 0);
+  const uint8_t *const files[1] = {file};
+  const size_t lens[1] = {len};
+  if ((rc = gst_pack_staging(files, lens, 1, staging.data(), staging.size())) != GST_OK) return rc;
+  hipStream_t s = nullptr;
+  GST_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  void *d_cmp = nullptr, *d_out = nullptr;
+  hipError_t e = hipMalloc(&d_cmp, staging.size());
+  if (e == hipSuccess) e = hipMalloc(&d_out, dxt);
+  if (e == hipSuccess) e = hipMemcpyAsync(d_cmp, staging.data(), staging.size(), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) {
+    rc = DecodeStaging(&h, 1, s, d_cmp, d_out, false, 0, nullptr, nullptr);
+    if (rc == GST_OK) e = hipMemcpyAsync(out, d_out, dxt, hipMemcpyDeviceToHost, s);
+    if (rc == GST_OK && e == hipSuccess) e = hipStreamSynchronize(s);
+  }
+  uint32_t bits = 0;
+  if (e == hipSuccess && rc == GST_OK) rc = gst_read_error_bits(s, &bits);
+  if (d_cmp) (void)hipFree(d_cmp);
+  if (d_out) (void)hipFree(d_out);
+  {
+    Globals &g = G();
+    std::lock_guard<std::mutex> lk(g.mu);
+    auto it = g.err_words.find(s);
+    if (it != g.err_words.end()) {
+      (void)hipFree(it->second);
+      g.err_words.erase(it);
+    }
+  }
+  (void)hipStreamDestroy(s);
+  if (e != hipSuccess) return HipFail(e);
+  if (rc != GST_OK) return rc;
+  return bits ? GST_ERR_CORRUPT_STREAM : GST_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batched decode of textures of uneven size (no reference counterpart; the
+// reference's LoadCompressedDXTs needs equal sizes, codec/decoder.cpp:105-121).
+
+struct gst_batch {
+  gst_batch_config cfg;
+  StageConfig stage;
+  uint32_t grids[gst::kNumStages];
+  uint8_t *dmem = nullptr;   // plan blob | tables | status | err | scratch
+  size_t dbytes = 0;
+  size_t plan_off = 0, tables_off = 0, status_off = 0, err_off = 0, scratch_off = 0;
+  size_t scratch_cap = 0;
+  void *h_blob = nullptr;    // pinned staging of the plan blob
+  size_t blob_bytes = 0;
+  hipEvent_t blob_ev = nullptr;
+  bool blob_pending = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  uint32_t n_bound = 0;
+};
+
+namespace {
+
+KernelArgs BatchArgs(gst_batch *b) {
+  return PlanArgs(b->dmem + b->plan_off, b->cfg.capacity_images,
+                  reinterpret_cast<uint32_t *>(b->dmem + b->err_off),
+                  reinterpret_cast<uint32_t *>(b->dmem + b->tables_off),
+                  reinterpret_cast<uint32_t *>(b->dmem + b->status_off));
+}
+
+}  // namespace
+
+extern "C" {
+
+int gst_batch_create(const gst_batch_config *cfg, gst_batch **out) {
+  if (!cfg || !out || cfg->capacity_images == 0 || cfg->capacity_blocks == 0)
+    return GST_ERR_INVALID_ARGUMENT;
+  int rc = EnsureInit();
+  if (rc != GST_OK) return rc;
+  gst_batch *b = new gst_batch();
+  b->cfg = *cfg;
+  b->stage.ans_groups_per_wave = cfg->ans_groups_per_wave ? cfg->ans_groups_per_wave : 2;
+  b->stage.ans_waves = cfg->ans_waves_per_group ? cfg->ans_waves_per_group : 4;
+  b->stage.rgb = cfg->rgb_output ? 1 : 0;
+  const uint32_t cus = static_cast<uint32_t>(G().cu_count > 0 ? G().cu_count : 256);
+  b->grids[gst::kStageTables] = std::min<uint32_t>(4 * cfg->capacity_images, cus);
+  b->grids[gst::kStageAns] = cfg->grid_ans ? cfg->grid_ans : cus * 8;
+  b->grids[gst::kStageIndices] = cfg->grid_indices ? cfg->grid_indices : cus * 4;
+  b->grids[gst::kStageWavelet] = cfg->grid_wavelet ? cfg->grid_wavelet : cus * 16;
+  b->grids[gst::kStageAssemble] = cfg->grid_assemble ? cfg->grid_assemble : cus * 8;
+  const uint64_t cap = cfg->capacity_images, B = cfg->capacity_blocks;
+  const uint64_t Pcap = cfg->capacity_palette ? cfg->capacity_palette : 4 * B;
+  size_t off = 0;
+  b->blob_bytes = PlanBytes(cfg->capacity_images);
+  b->plan_off = off; off += AlignUp(b->blob_bytes, kAlign);
+  b->tables_off = off; off += AlignUp(4 * cap * gst::kTableSize * 4, kAlign);
+  b->status_off = off; off += AlignUp(4 * cap * 4, kAlign);
+  b->err_off = off; off += kAlign;
+  b->scratch_off = off;
+  // Per image: symbols 7nv+P, planes 6nv, indices 4nv, group sums nv/2048 bytes,
+  // each region padded to 256 B.
+  b->scratch_cap = AlignUp(17 * B + Pcap + B / 2048 + cap * 4 * kAlign, kAlign);
+  off += b->scratch_cap;
+  b->dbytes = off;
+  hipError_t e = hipMalloc(&b->dmem, b->dbytes);
+  if (e == hipSuccess) e = hipMemset(b->dmem, 0, b->scratch_off);
+  if (e == hipSuccess) e = hipHostMalloc(&b->h_blob, b->blob_bytes, hipHostMallocDefault);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->blob_ev, hipEventDisableTiming);
+  if (e != hipSuccess) {
+    gst_batch_destroy(b);
+    return HipFail(e);
+  }
+  *out = b;
+  return GST_OK;
+}
+
+int gst_batch_bind(gst_batch *b, const gst_header *hdrs, uint32_t n, const void *const *d_files,
+                   void *const *d_outs, gst_stream stream) {
+  if (!b || !hdrs || !d_files || !d_outs || n == 0 || n > b->cfg.capacity_images)
+    return GST_ERR_INVALID_ARGUMENT;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  uint64_t blocks = 0, pal = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    const int rc = CheckHeader(hdrs[i]);
+    if (rc != GST_OK) return rc;
+    if (!d_files[i] || !d_outs[i]) return GST_ERR_INVALID_ARGUMENT;
+    blocks += NumBlocks(hdrs[i]);
+    pal += hdrs[i].palette_bytes;
+  }
+  const uint64_t Pcap = b->cfg.capacity_palette ? b->cfg.capacity_palette : 4 * b->cfg.capacity_blocks;
+  if (blocks > b->cfg.capacity_blocks || pal > Pcap) return GST_ERR_OUT_OF_MEMORY;
+  std::vector<ImageDesc> descs(n);
+  uint8_t *scratch = b->dmem + b->scratch_off;
+  size_t off = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    const Carve c = CarveFor(hdrs[i]);
+    if (off + c.total > b->scratch_cap) return GST_ERR_OUT_OF_MEMORY;
+    const uint8_t *f = static_cast<const uint8_t *>(d_files[i]);
+    FillDesc(hdrs[i], f + sizeof(gst_header), f + sizeof(gst_header) + 2048, scratch + off, c, 4 * i,
+             static_cast<uint8_t *>(d_outs[i]), b->stage.rgb != 0, &descs[i]);
+    off += c.total;
+  }
+  std::vector<uint8_t> blob;
+  uint32_t units[gst::kNumStages];
+  BuildPlan(descs, b->cfg.capacity_images, GroupsPerUnit(b->stage), &blob, units);
+  if (b->blob_pending) GST_HIP(hipEventSynchronize(b->blob_ev));
+  memcpy(b->h_blob, blob.data(), blob.size());
+  GST_HIP(hipMemcpyAsync(b->dmem + b->plan_off, b->h_blob, blob.size(), hipMemcpyHostToDevice, s));
+  GST_HIP(hipEventRecord(b->blob_ev, s));
+  b->blob_pending = true;
+  b->n_bound = n;
+  return GST_OK;
+}
+
+int gst_batch_decode(gst_batch *b, gst_stream stream) {
+  if (!b || b->n_bound == 0) return GST_ERR_INVALID_ARGUMENT;
+  return LaunchAll(BatchArgs(b), b->grids, b->stage, static_cast<hipStream_t>(stream));
+}
+
+int gst_batch_capture_graph(gst_batch *b, gst_stream stream) {
+  if (!b || b->n_bound == 0) return GST_ERR_INVALID_ARGUMENT;
+  if (b->exec) return GST_OK;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  GST_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  const int rc = LaunchAll(BatchArgs(b), b->grids, b->stage, s);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(s, &g);
+  if (rc != GST_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  GST_HIP(e);
+  b->graph = g;
+  GST_HIP(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+  return GST_OK;
+}
+
+int gst_batch_launch_graph(gst_batch *b, gst_stream stream) {
+  if (!b || !b->exec || b->n_bound == 0) return GST_ERR_INVALID_ARGUMENT;
+  GST_HIP(hipGraphLaunch(b->exec, static_cast<hipStream_t>(stream)));
+  return GST_OK;
+}
+
+int gst_batch_decode_timed(gst_batch *b, gst_stream stream, float ms[5]) {
+  if (!b || !ms || b->n_bound == 0) return GST_ERR_INVALID_ARGUMENT;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipEvent_t ev[gst::kNumStages + 1];
+  for (auto &e : ev) GST_HIP(hipEventCreate(&e));
+  const KernelArgs a = BatchArgs(b);
+  int rc = GST_OK;
+  hipError_t e = hipEventRecord(ev[0], s);
+  for (uint32_t st = 0; st < gst::kNumStages && e == hipSuccess; ++st) {
+    e = gst::LaunchStage(st, a, b->grids[st], b->stage, s);
+    if (e == hipSuccess) e = hipEventRecord(ev[st + 1], s);
+  }
+  if (e == hipSuccess) e = hipEventSynchronize(ev[gst::kNumStages]);
+  for (uint32_t st = 0; st < gst::kNumStages && e == hipSuccess; ++st)
+    e = hipEventElapsedTime(&ms[st], ev[st], ev[st + 1]);
+  if (e != hipSuccess) rc = HipFail(e);
+  for (auto &x : ev) (void)hipEventDestroy(x);
+  return rc;
+}
+
+int gst_batch_error_bits(gst_batch *b, gst_stream stream, uint32_t *bits) {
+  if (!b || !bits) return GST_ERR_INVALID_ARGUMENT;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  GST_HIP(hipStreamSynchronize(s));
+  GST_HIP(hipMemcpy(bits, b->dmem + b->err_off, sizeof(uint32_t), hipMemcpyDeviceToHost));
+  GST_HIP(hipMemset(b->dmem + b->err_off, 0, sizeof(uint32_t)));
+  return GST_OK;
+}
+
+size_t gst_batch_device_bytes(const gst_batch *b) { return b ? b->dbytes : 0; }
+
+int gst_batch_destroy(gst_batch *b) {
+  if (!b) return GST_ERR_INVALID_ARGUMENT;
+  if (b->dmem) (void)hipDeviceSynchronize();
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->blob_ev) (void)hipEventDestroy(b->blob_ev);
+  if (b->h_blob) (void)hipHostFree(b->h_blob);
+  if (b->dmem) (void)hipFree(b->dmem);
+  delete b;
+  return GST_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// One launch per stage, single image.
+
+namespace {
+
+int StageLaunch(uint32_t stage, const gst_header *hdr, const uint8_t *d_freqs, const uint8_t *d_streams,
+                const uint32_t *d_tables, const uint8_t *d_symbols, const int32_t *d_group_sums,
+                const int8_t *d_planes, const int32_t *d_indices, void *d_out, uint32_t *d_err,
+                bool rgb, hipStream_t s) {
+  int rc = EnsureInit();
+  if (rc != GST_OK) return rc;
+  gst_header h;
+  if (hdr) {
+    h = *hdr;
+    if ((rc = CheckHeader(h)) != GST_OK) return rc;
+  } else {
+    // build_table needs no geometry; any legal header keeps the descriptor sane.
+    h = gst_header{512, 256, 8192, 4 * 4 + 4 * 128, 8 * 4 + 8 * 128, 4 + 128, 4 + 128};
+  }
+  Carve c = CarveFor(h);
+  ImageDesc d;
+  FillDesc(h, d_freqs, d_streams ? d_streams : d_freqs, const_cast<uint8_t *>(d_symbols) - c.symbols, c,
+           0, static_cast<uint8_t *>(d_out), rgb, &d);
+  d.symbols = const_cast<uint8_t *>(d_symbols);
+  d.planes = const_cast<int8_t *>(d_planes);
+  d.indices = const_cast<int32_t *>(d_indices);
+  d.group_sums = const_cast<int32_t *>(d_group_sums);
+  KernelArgs a;
+  memset(&a, 0, sizeof(a));
+  a.single = d;
+  a.err = d_err;
+  a.tables = const_cast<uint32_t *>(d_tables);
+  a.table_status = const_cast<uint32_t *>(d_tables) + 4 * gst::kTableSize;
+  const StageConfig cfg = DefaultStageConfig(rgb);
+  const uint32_t grid = gst::StageUnitsOf(d, stage, GroupsPerUnit(cfg));
+  GST_HIP(gst::LaunchStage(stage, a, grid, cfg, s));
+  return GST_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gst_stage_build_table(const void *d_freqs, uint32_t *d_tables, uint32_t *d_err, gst_stream stream) {
+  if (!d_freqs || !d_tables || !d_err) return GST_ERR_INVALID_ARGUMENT;
+  return StageLaunch(gst::kStageTables, nullptr, static_cast<const uint8_t *>(d_freqs), nullptr,
+                     d_tables, nullptr, nullptr, nullptr, nullptr, nullptr, d_err, false,
+                     static_cast<hipStream_t>(stream));
+}
+
+int gst_stage_ans_decode(const gst_header *hdr, const void *d_streams, const uint32_t *d_tables,
+                         uint8_t *d_symbols, int32_t *d_group_sums, uint32_t *d_err, gst_stream stream) {
+  if (!hdr || !d_streams || !d_tables || !d_symbols || !d_group_sums || !d_err)
+    return GST_ERR_INVALID_ARGUMENT;
+  return StageLaunch(gst::kStageAns, hdr, nullptr, static_cast<const uint8_t *>(d_streams), d_tables,
+                     d_symbols, d_group_sums, nullptr, nullptr, nullptr, d_err, false,
+                     static_cast<hipStream_t>(stream));
+}
+
+int gst_stage_decode_indices(const gst_header *hdr, const uint8_t *d_symbols, const int32_t *d_group_sums,
+                             int32_t *d_indices, gst_stream stream) {
+  if (!hdr || !d_symbols || !d_group_sums || !d_indices) return GST_ERR_INVALID_ARGUMENT;
+  return StageLaunch(gst::kStageIndices, hdr, nullptr, nullptr, nullptr, d_symbols, d_group_sums,
+                     nullptr, d_indices, nullptr, nullptr, false, static_cast<hipStream_t>(stream));
+}
+
+int gst_stage_inv_wavelet(const gst_header *hdr, const uint8_t *d_symbols, int8_t *d_planes,
+                          gst_stream stream) {
+  if (!hdr || !d_symbols || !d_planes) return GST_ERR_INVALID_ARGUMENT;
+  return StageLaunch(gst::kStageWavelet, hdr, nullptr, nullptr, nullptr, d_symbols, nullptr, d_planes,
+                     nullptr, nullptr, nullptr, false, static_cast<hipStream_t>(stream));
+}
+
+int gst_stage_assemble_dxt(const gst_header *hdr, const uint8_t *d_symbols, const int8_t *d_planes,
+                           const int32_t *d_indices, void *d_out, uint32_t *d_err, gst_stream stream) {
+  if (!hdr || !d_symbols || !d_planes || !d_indices || !d_out || !d_err) return GST_ERR_INVALID_ARGUMENT;
+  return StageLaunch(gst::kStageAssemble, hdr, nullptr, nullptr, nullptr, d_symbols, nullptr, d_planes,
+                     d_indices, d_out, d_err, false, static_cast<hipStream_t>(stream));
+}
+
+int gst_stage_assemble_rgb(const gst_header *hdr, const uint8_t *d_symbols, const int8_t *d_planes,
+                           const int32_t *d_indices, void *d_out, uint32_t *d_err, gst_stream stream) {
+  if (!hdr || !d_symbols || !d_planes || !d_indices || !d_out || !d_err) return GST_ERR_INVALID_ARGUMENT;
+  return StageLaunch(gst::kStageAssemble, hdr, nullptr, nullptr, nullptr, d_symbols, nullptr, d_planes,
+                     d_indices, d_out, d_err, true, static_cast<hipStream_t>(stream));
+}
+
+}  // extern "C"

@@ -1,0 +1,109 @@
+// gst_internal.h -- descriptors shared by the host orchestration and the gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace gst {
+
+constexpr uint32_t kTableSize = 2048;        // M (ans/ans_ocl_encode.cpp:10-18)
+constexpr uint32_t kTableLog = 11;
+constexpr uint32_t kStateL = 32768;          // k*M (ans/ans_decode.cl:6-8)
+constexpr uint32_t kLanesPerGroup = 32;      // streams per rANS group (ans/ans.h:75)
+constexpr uint32_t kSymsPerLane = 256;       // ans/ans.h:74
+constexpr uint32_t kGroupSyms = kLanesPerGroup * kSymsPerLane;
+constexpr uint32_t kTileDim = 32;            // codec/codec_base.h:22
+constexpr uint32_t kTileSyms = kTileDim * kTileDim;
+constexpr uint32_t kAssembleBlocksPerUnit = 1024;
+constexpr uint32_t kWaveletTilesPerUnit = 2;  // one wave64, a tile per half-wave
+
+enum Stage : uint32_t {
+  kStageTables = 0,
+  kStageAns = 1,
+  kStageIndices = 2,
+  kStageWavelet = 3,
+  kStageAssemble = 4,
+  kNumStages = 5
+};
+
+enum DevErr : uint32_t {
+  kErrStreamOffsets = 0x1,
+  kErrFrequencies = 0x2,
+  kErrPaletteIndex = 0x4,
+};
+
+// Per-image descriptor.  Stream k: 0 = Y, 1 = chroma, 2 = palette, 3 = indices
+// (codec/encoder.cpp:122-144).  Symbol buffer layout (codec/decoder.cpp:441-461):
+// [Y 2nv][chroma 4nv][palette P][indices nv]; planes p = Y1,Y2,Co1,Cg1,Co2,Cg2 are the
+// first 6nv symbols (codec/assemble.cl:303-313).
+struct ImageDesc {
+  uint32_t width, height, nv, bx;
+  uint32_t palette_bytes;
+  uint32_t groups[4];
+  uint32_t stream_len[4];
+  uint32_t table_base;   // index of the image's first of 4 tables
+  uint32_t rgb;          // 1: RGB8 output
+  const uint8_t *freqs;  // 4 x 256 little-endian u16
+  const uint8_t *streams[4];
+  uint8_t *symbols;
+  int8_t *planes;
+  int32_t *indices;
+  int32_t *group_sums;   // one per index-stream group
+  uint8_t *out;
+};
+
+__host__ __device__ inline uint32_t SymbolOffset(const ImageDesc &d, uint32_t k) {
+  return k == 0 ? 0u : k == 1 ? 2 * d.nv : k == 2 ? 6 * d.nv : 6 * d.nv + d.palette_bytes;
+}
+
+__host__ __device__ inline uint32_t AnsUnitsOf(const ImageDesc &d, uint32_t groups_per_unit) {
+  uint32_t u = 0;
+  for (uint32_t k = 0; k < 4; ++k) u += (d.groups[k] + groups_per_unit - 1) / groups_per_unit;
+  return u;
+}
+
+__host__ __device__ inline uint32_t StageUnitsOf(const ImageDesc &d, uint32_t stage,
+                                                 uint32_t groups_per_unit) {
+  switch (stage) {
+    case kStageTables: return 4;
+    case kStageAns: return AnsUnitsOf(d, groups_per_unit);
+    case kStageIndices: return d.nv / kGroupSyms;
+    case kStageWavelet: return 6 * (d.nv / kTileSyms) / kWaveletTilesPerUnit;
+    default: return d.nv / kAssembleBlocksPerUnit;
+  }
+}
+
+// Device plan for batched decodes: header, then ImageDesc[capacity], then
+// prefix[kNumStages][capacity + 1] (work units per stage, exclusive prefix over images).
+struct PlanHeader {
+  uint32_t n_images;
+  uint32_t capacity;
+  uint32_t units[kNumStages];
+  uint32_t groups_per_unit;
+};
+
+// Kernel arguments (by value).  plan == nullptr selects the single image in `single`.
+struct KernelArgs {
+  const PlanHeader *plan;
+  const ImageDesc *images;
+  const uint32_t *prefix;
+  uint32_t *err;
+  uint32_t *tables;        // n_tables x 2048 packed entries
+  uint32_t *table_status;  // n_tables: 1 = valid
+  ImageDesc single;
+};
+
+// Launch shape of the rANS stage and the assembly flavour.
+struct StageConfig {
+  uint32_t ans_waves;            // waves per rANS work-group (1, 2, 4, 8)
+  uint32_t ans_groups_per_wave;  // 1 or 2
+  uint32_t rgb;                  // 1: RGB8 assembly
+};
+
+// Launches one stage with `grid` work-groups (each loops over units grid-stride).
+hipError_t LaunchStage(uint32_t stage, const KernelArgs &args, uint32_t grid,
+                       const StageConfig &cfg, hipStream_t stream);
+
+}  // namespace gst

@@ -1,0 +1,579 @@
+// gst_kernels.hip -- the five GST decode stages as gfx950 kernels (wave64, integer only).
+//
+// Every kernel is a grid-stride loop over "units" of its stage; a unit maps to
+// (image, local unit) through the batch plan's per-stage prefix array, or to the
+// single image passed by value.  Reference semantics are cited per kernel.
+#include "gst_internal.h"
+
+namespace gst {
+namespace {
+
+constexpr int32_t kBufferRsrcWord3 = 0x00020000;  // raw buffer, gfx9 family
+
+__device__ __forceinline__ uint32_t StageTotal(const KernelArgs &a, uint32_t stage,
+                                               uint32_t groups_per_unit) {
+  return a.plan ? a.plan->units[stage] : StageUnitsOf(a.single, stage, groups_per_unit);
+}
+
+struct Located {
+  uint32_t img;
+  uint32_t local;
+};
+
+// Largest image i with prefix[i] <= u (every image has at least one unit per stage).
+__device__ __forceinline__ Located Locate(const KernelArgs &a, uint32_t stage, uint32_t u) {
+  if (!a.plan) return {0u, u};
+  const uint32_t n = a.plan->n_images;
+  const uint32_t *pre = a.prefix + stage * (a.plan->capacity + 1);
+  uint32_t lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (pre[mid] <= u) lo = mid; else hi = mid;
+  }
+  return {lo, u - pre[lo]};
+}
+
+__device__ __forceinline__ ImageDesc LoadDesc(const KernelArgs &a, uint32_t img) {
+  return a.plan ? a.images[img] : a.single;
+}
+
+// Element k of a 4-entry descriptor array without dynamic indexing (which would put
+// the descriptor copy in scratch memory).
+template <typename T>
+__device__ __forceinline__ T Pick4(const T (&v)[4], uint32_t k) {
+  return k == 0 ? v[0] : k == 1 ? v[1] : k == 2 ? v[2] : v[3];
+}
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t MakeRsrc(const void *base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), 0, static_cast<int>(bytes),
+                                           kBufferRsrcWord3);
+}
+
+template <typename T>
+__device__ __forceinline__ T WaveSumWidth32(T v) {
+#pragma unroll
+  for (int off = 16; off > 0; off >>= 1) v += __shfl_xor(v, off, 32);
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// Stage 1: rANS slot tables (ans/build_table.cl:12-83).
+//
+// One work-group per table: a 256-entry scan of the frequencies, then every slot s
+// takes the symbol x with cum[x] <= s < cum[x] + f[x].  The reference finds x with an
+// 11-step branch-free search; for tables that sum to M its result is that closed
+// form (the search path depends only on the target symbol, checked exhaustively
+// against the oracle).  Entry packing, 4 bytes:
+//   bits 0-11 f[x] (1..2048), bits 12-22 s - cum[x], bits 24-31 x.
+// A table that does not sum to 2048 is flagged in table_status and replaced by the
+// identity table (f = 2048, offset = slot, symbol 0), which never renormalises.
+__global__ __launch_bounds__(256) void BuildTableKernel(KernelArgs a) {
+  __shared__ uint32_t cum[256];
+  __shared__ uint32_t freq[256];
+  __shared__ uint32_t wsum[4];
+  const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const uint32_t total = a.plan ? a.plan->n_images * 4 : 4;
+  for (uint32_t u = blockIdx.x; u < total; u += gridDim.x) {
+    const uint32_t img = u >> 2, k = u & 3;
+    const ImageDesc d = LoadDesc(a, img);
+    const uint32_t tix = d.table_base + k;
+    const uint16_t *fr = reinterpret_cast<const uint16_t *>(d.freqs + 512 * k);
+    const uint32_t f = fr[t];
+    uint32_t v = f;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t n = __shfl_up(v, off);
+      if (lane >= static_cast<uint32_t>(off)) v += n;
+    }
+    if (lane == 63) wsum[wave] = v;
+    __syncthreads();
+    uint32_t add = 0;
+    for (uint32_t w = 0; w < wave; ++w) add += wsum[w];
+    cum[t] = v + add - f;
+    freq[t] = f;
+    const uint32_t sum = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    __syncthreads();
+    const bool valid = sum == kTableSize;
+    uint32_t *tab = a.tables + static_cast<size_t>(tix) * kTableSize;
+    for (uint32_t slot = t; slot < kTableSize; slot += 256) {
+      uint32_t e;
+      if (valid) {
+        uint32_t x;
+        if (cum[255] <= slot) {
+          x = 255;
+        } else {
+          uint32_t pos = 0;  // number of cum[0..254] <= slot
+#pragma unroll
+          for (uint32_t s = 128; s; s >>= 1)
+            if (cum[pos + s - 1] <= slot) pos += s;
+          x = pos - 1;
+        }
+        e = freq[x] | ((slot - cum[x]) << 12) | (x << 24);
+      } else {
+        e = kTableSize | (slot << 12);
+      }
+      tab[slot] = e;
+    }
+    if (t == 0) a.table_status[tix] = valid ? 1u : 0u;
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Stage 2: interleaved rANS decode (ans/ans_decode.cl:97-157).
+//
+// A work-group of WAVES wave64s decodes GPW*WAVES consecutive groups of one stream
+// with that stream's table resident in LDS, widened to 8 bytes per slot
+// {f | symbol << 24, slot - cum} so the state update is one v_mad_u32_u24.  With
+// GPW = 2 a wave carries two 32-stream groups, one per half-wave; each lane holds
+// one stream's state.  The reference's local-memory renormalisation mask
+// (atomic_or / popcount / atomic_and between barriers) becomes one ballot: a lane
+// that needs a word reads u16[p_half - total_in_half + rank_in_half], the rank
+// coming from mbcnt over the 64-bit ballot, and each half-wave's pointer (a scalar)
+// moves down by its total.  All stream reads go through a buffer resource bounded
+// by the stream length, so corrupt offsets read zeros instead of faulting; a
+// pointer that ends below the group's first byte raises kErrStreamOffsets.  Each
+// lane packs 16 symbols into four registers and writes them with one 16-byte
+// store, in the reference's output order (lane j, step i of the group at
+// j*256 + 255 - i).  Index-stream groups also emit the sum of (symbol - 128), the
+// per-group total the index scan starts from.
+template <bool kSum>
+__device__ __forceinline__ void DecodeHalfGroups(const uint2 *table, __amdgpu_buffer_rsrc_t rs,
+                                                 uint32_t lane, bool active, uint32_t x,
+                                                 uint32_t pA, uint32_t pB, uint32_t loA,
+                                                 uint32_t loB, uint8_t *outp, int32_t *group_sum,
+                                                 uint32_t *err) {
+  uint32_t upper = lane < 32 ? 0u : 0xFFFFFFFFu;
+  asm volatile("" : "+v"(upper));  // keep the half-wave mask an opaque VGPR (one v_and below)
+  const uint64_t active_mask = __builtin_amdgcn_ballot_w64(active);
+  int32_t sum = 0;
+  for (uint32_t blk = 0; blk < kSymsPerLane / 16; ++blk) {
+    uint32_t acc0 = 0, acc1 = 0, acc2 = 0, acc3 = 0;
+#pragma unroll
+    for (uint32_t s = 0; s < 16; ++s) {
+      const uint2 e = table[x & (kTableSize - 1)];
+      x = __umul24(x >> kTableLog, e.x) + e.y;  // (x >> 11) * f + slot - cum (low 24 bits of e.x = f)
+      const uint64_t m = __builtin_amdgcn_uicmp(x, kStateL, 36 /* ULT */) & active_mask;
+      const uint32_t mlo = static_cast<uint32_t>(m), mhi = static_cast<uint32_t>(m >> 32);
+      const uint32_t tlo = __popc(mlo), thi = __popc(mhi);
+      pA -= tlo;
+      pB -= thi;
+      if (active && x < kStateL) {
+        // Byte offset of u16[p_half + rank]: mbcnt over both halves counts the lower
+        // half's bits for upper lanes, so upper lanes start from pB - tlo - pA.
+        const uint32_t t = upper & (pB - tlo - pA);
+        const uint32_t widx = __builtin_amdgcn_mbcnt_hi(mhi, __builtin_amdgcn_mbcnt_lo(mlo, t));
+        const uint32_t w = __builtin_amdgcn_raw_buffer_load_b16(rs, static_cast<int>(widx * 2 + pA * 2), 0, 0);
+        x = (x << 16) | w;
+      }
+      const uint32_t sym = e.x >> 24;
+      const uint32_t pos = 15 - s;
+      const uint32_t bits = sym << (8 * (pos & 3));
+      if ((pos >> 2) == 0) acc0 |= bits;
+      else if ((pos >> 2) == 1) acc1 |= bits;
+      else if ((pos >> 2) == 2) acc2 |= bits;
+      else acc3 |= bits;
+      if (kSum) sum += static_cast<int32_t>(sym) - 128;
+    }
+    if (active)
+      *reinterpret_cast<uint4 *>(outp + (kSymsPerLane - 16) - 16 * blk) =
+          make_uint4(acc0, acc1, acc2, acc3);
+  }
+  const bool under = lane < 32 ? (pA < loA) : (pB < loB);
+  if (active && under && (lane & 31) == 0) atomicOr(err, kErrStreamOffsets);
+  if (kSum) {
+    sum = WaveSumWidth32(sum);
+    if (active && (lane & 31) == 0) *group_sum = sum;
+  }
+}
+
+__device__ __forceinline__ uint32_t Uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ __forceinline__ const uint8_t *Uniform(const uint8_t *p) {
+  const uint64_t v = reinterpret_cast<uint64_t>(p);
+  const uint64_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v));
+  const uint64_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v >> 32));
+  return reinterpret_cast<const uint8_t *>(lo | (hi << 32));
+}
+
+template <uint32_t WAVES, uint32_t GPW>
+__global__ __launch_bounds__(WAVES * 64) void AnsDecodeKernel(KernelArgs a) {
+  __shared__ uint2 table[kTableSize];
+  constexpr uint32_t kGroupsPerUnit = WAVES * GPW;
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t half = GPW == 2 ? (lane >> 5) : 0;
+  const uint32_t j = lane & 31;
+  const uint32_t total = StageTotal(a, kStageAns, kGroupsPerUnit);
+  for (uint32_t u = blockIdx.x; u < total; u += gridDim.x) {
+    const Located loc = Locate(a, kStageAns, u);
+    const ImageDesc d = LoadDesc(a, loc.img);
+    uint32_t k = 0, local = loc.local;
+    for (; k < 3; ++k) {
+      const uint32_t uk = (Pick4(d.groups, k) + kGroupsPerUnit - 1) / kGroupsPerUnit;
+      if (local < uk) break;
+      local -= uk;
+    }
+    k = Uniform(k);
+    local = Uniform(local);
+    const uint32_t tix = Uniform(d.table_base + k);
+    __syncthreads();
+    {
+      const uint32_t *src = a.tables + static_cast<size_t>(tix) * kTableSize;
+      for (uint32_t i = threadIdx.x; i < kTableSize; i += WAVES * 64) {
+        const uint32_t e = src[i];
+        table[i] = make_uint2((e & 0xFFFu) | (e & 0xFF000000u), (e >> 12) & 0x7FFu);
+      }
+    }
+    const bool table_ok = a.table_status[tix] != 0;
+    __syncthreads();
+    if (!table_ok && threadIdx.x == 0) atomicOr(a.err, kErrFrequencies);
+
+    const uint32_t G = Uniform(Pick4(d.groups, k));
+    const uint32_t g = local * kGroupsPerUnit + wave * GPW + half;
+    const uint32_t slen = Uniform(Pick4(d.stream_len, k));
+    const __amdgpu_buffer_rsrc_t rs = MakeRsrc(Uniform(Pick4(d.streams, k)), slen);
+    bool active = (GPW == 2 || lane < 32) && g < G;
+    uint32_t end = 0, lo = 0;
+    if (active) {
+      end = __builtin_amdgcn_raw_buffer_load_b32(rs, static_cast<int>(4 * g), 0, 0);
+      lo = g == 0 ? 4 * G : __builtin_amdgcn_raw_buffer_load_b32(rs, static_cast<int>(4 * (g - 1)), 0, 0);
+      const bool ok = ((end | lo) & 3) == 0 && end <= slen && lo >= 4 * G && lo <= end &&
+                      end - lo >= 4 * kLanesPerGroup;
+      if (!ok) {
+        if (j == 0) atomicOr(a.err, kErrStreamOffsets);
+        active = false;
+      }
+    }
+    const uint32_t x = active
+        ? __builtin_amdgcn_raw_buffer_load_b32(rs, static_cast<int>(end - 4 * kLanesPerGroup + 4 * j), 0, 0)
+        : 0u;
+    const uint32_t p = active ? (end - 4 * kLanesPerGroup) >> 1 : 0u;
+    const uint32_t plo = active ? lo >> 1 : 0u;
+    const uint32_t pA = __builtin_amdgcn_readlane(p, 0);
+    const uint32_t loA = __builtin_amdgcn_readlane(plo, 0);
+    const uint32_t pB = GPW == 2 ? __builtin_amdgcn_readlane(p, 32) : 0u;
+    const uint32_t loB = GPW == 2 ? __builtin_amdgcn_readlane(plo, 32) : 0u;
+    uint8_t *outp = d.symbols + SymbolOffset(d, k) + static_cast<size_t>(g) * kGroupSyms + j * kSymsPerLane;
+    if (k == 3)
+      DecodeHalfGroups<true>(table, rs, lane, active, x, pA, pB, loA, loB, outp,
+                             d.group_sums + g, a.err);
+    else
+      DecodeHalfGroups<false>(table, rs, lane, active, x, pA, pB, loA, loB, outp, nullptr, a.err);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Stage 3: palette indices (codec/decode_indices.cl:6-84 composed by
+// codec/decoder.cpp:304-393): idx[i] = sum_{j<=i} (diff[j] - 128), int32.
+//
+// One work-group per 8192-entry index group.  The group's starting value is the
+// sum of the preceding groups' totals, which the rANS stage already produced, so
+// the whole scan is one launch with no inter-work-group traffic.  Each thread
+// scans 32 entries in registers, then wave and work-group scans join them.
+__global__ __launch_bounds__(256) void DecodeIndicesKernel(KernelArgs a) {
+  __shared__ uint32_t wtot[4];
+  __shared__ uint32_t gprefix;
+  const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const uint32_t total = StageTotal(a, kStageIndices, 1);
+  for (uint32_t u = blockIdx.x; u < total; u += gridDim.x) {
+    const Located loc = Locate(a, kStageIndices, u);
+    const ImageDesc d = LoadDesc(a, loc.img);
+    const uint32_t g = loc.local;
+    if (wave == 0) {
+      uint32_t s = 0;
+      for (uint32_t i = lane; i < g; i += 64) s += static_cast<uint32_t>(d.group_sums[i]);
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+      if (lane == 0) gprefix = s;
+    }
+    const uint8_t *diffs = d.symbols + SymbolOffset(d, 3) + static_cast<size_t>(g) * kGroupSyms + 32 * t;
+    const uint4 q0 = reinterpret_cast<const uint4 *>(diffs)[0];
+    const uint4 q1 = reinterpret_cast<const uint4 *>(diffs)[1];
+    const uint32_t words[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
+    uint32_t run[32];
+    uint32_t acc = 0;
+#pragma unroll
+    for (int i = 0; i < 32; ++i) {
+      acc += ((words[i >> 2] >> (8 * (i & 3))) & 0xFFu) - 128u;
+      run[i] = acc;
+    }
+    uint32_t incl = acc;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t n = __shfl_up(incl, off);
+      if (lane >= static_cast<uint32_t>(off)) incl += n;
+    }
+    if (lane == 63) wtot[wave] = incl;
+    __syncthreads();
+    uint32_t offset = gprefix + incl - acc;
+    for (uint32_t w = 0; w < wave; ++w) offset += wtot[w];
+    uint4 *dst = reinterpret_cast<uint4 *>(d.indices + static_cast<size_t>(g) * kGroupSyms + 32 * t);
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+      dst[i] = make_uint4(run[4 * i] + offset, run[4 * i + 1] + offset, run[4 * i + 2] + offset,
+                          run[4 * i + 3] + offset);
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Stage 4: inverse 5/3 wavelet (codec/inverse_wavelet.cl:98-276).
+//
+// One wave64 per two 32x32 tiles, one tile per half-wave.  A lane holds a whole
+// row of its tile in 32 registers, so the row pass of each level is straight-line
+// code with compile-time mirror indices; the column pass runs after a transpose
+// through LDS (row stride 33 words: conflict-free in both directions).  Levels
+// len = 2..32 each transform rows y < len over x < len, then columns x < len over
+// y < len, exactly the reference's transposed ping-pong.  The last level leaves
+// lane r holding column r, which is stored as int8 with 32-byte row segments.
+__host__ __device__ constexpr int Mirror(int idx, int range) {
+  return (idx - static_cast<int>(idx >= range) * (idx - range + 2)) < 0
+             ? -(idx - static_cast<int>(idx >= range) * (idx - range + 2))
+             : (idx - static_cast<int>(idx >= range) * (idx - range + 2));
+}
+
+template <int LEN>
+__device__ __forceinline__ void Inverse1D(int (&v)[kTileDim]) {
+  constexpr int MID = LEN / 2;
+  int e[LEN];
+#pragma unroll
+  for (int k = 0; k < MID; ++k) {
+    const int p = MID + Mirror(2 * k - 1, LEN) / 2;
+    const int n = MID + Mirror(2 * k + 1, LEN) / 2;
+    e[2 * k] = v[k] - (v[p] + v[n] + 2) / 4;
+  }
+#pragma unroll
+  for (int k = 0; k < MID; ++k) {
+    const int p = Mirror(2 * k, LEN);
+    const int n = Mirror(2 * k + 2, LEN);
+    e[2 * k + 1] = v[MID + k] + (e[p] + e[n]) / 2;
+  }
+#pragma unroll
+  for (int i = 0; i < LEN; ++i) v[i] = e[i];
+}
+
+constexpr int kLdsStride = kTileDim + 1;
+
+template <int LEN>
+__device__ __forceinline__ void WaveletLevel(int (&v)[kTileDim], int *L, int r) {
+  if (r < LEN) {
+    Inverse1D<LEN>(v);
+#pragma unroll
+    for (int c = 0; c < LEN; ++c) L[r * kLdsStride + c] = v[c];
+  }
+  __syncthreads();
+  if (r < LEN) {
+#pragma unroll
+    for (int c = 0; c < LEN; ++c) v[c] = L[c * kLdsStride + r];
+  }
+  __syncthreads();
+  if (r < LEN) Inverse1D<LEN>(v);
+  if (LEN < static_cast<int>(kTileDim)) {
+    if (r < LEN) {
+#pragma unroll
+      for (int c = 0; c < LEN; ++c) L[c * kLdsStride + r] = v[c];
+    }
+    __syncthreads();
+    if (r < LEN) {
+#pragma unroll
+      for (int c = 0; c < LEN; ++c) v[c] = L[r * kLdsStride + c];
+    }
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(64) void InvWaveletKernel(KernelArgs a) {
+  __shared__ int lds[2][kTileDim * kLdsStride];
+  const uint32_t lane = threadIdx.x, half = lane >> 5, r = lane & 31;
+  const uint32_t total = StageTotal(a, kStageWavelet, 1);
+  for (uint32_t u = blockIdx.x; u < total; u += gridDim.x) {
+    const Located loc = Locate(a, kStageWavelet, u);
+    const ImageDesc d = LoadDesc(a, loc.img);
+    const uint32_t tiles_per_plane = d.nv / kTileSyms;
+    const uint32_t tile = loc.local * kWaveletTilesPerUnit + half;
+    const uint32_t plane = tile / tiles_per_plane, t = tile % tiles_per_plane;
+    const uint8_t *src = d.symbols + static_cast<size_t>(plane) * d.nv + static_cast<size_t>(t) * kTileSyms + r * kTileDim;
+    const uint4 q0 = reinterpret_cast<const uint4 *>(src)[0];
+    const uint4 q1 = reinterpret_cast<const uint4 *>(src)[1];
+    const uint32_t words[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
+    int v[kTileDim];
+#pragma unroll
+    for (int i = 0; i < static_cast<int>(kTileDim); ++i)
+      v[i] = static_cast<int>((words[i >> 2] >> (8 * (i & 3))) & 0xFFu) - 128;
+    int *L = lds[half];
+    WaveletLevel<2>(v, L, static_cast<int>(r));
+    WaveletLevel<4>(v, L, static_cast<int>(r));
+    WaveletLevel<8>(v, L, static_cast<int>(r));
+    WaveletLevel<16>(v, L, static_cast<int>(r));
+    WaveletLevel<32>(v, L, static_cast<int>(r));
+    const uint32_t tiles_x = d.bx / kTileDim;
+    const uint32_t tx = t % tiles_x, ty = t / tiles_x;
+    int8_t *dst = d.planes + static_cast<size_t>(plane) * d.nv +
+                  static_cast<size_t>(ty * kTileDim) * d.bx + tx * kTileDim + r;
+#pragma unroll
+    for (int y = 0; y < static_cast<int>(kTileDim); ++y)
+      dst[static_cast<size_t>(y) * d.bx] = static_cast<int8_t>(v[y]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Stage 5: DXT1 assembly (codec/assemble.cl:290-357) and RGB8 assembly (:359-406).
+//
+// A thread assembles 4 consecutive blocks: one 4-byte load per plane, one 16-byte
+// index load, 4 palette gathers and two 16-byte stores.  YCoCg -> RGB565 uses C
+// truncating division; the pack keeps the low 16 bits of the int shifts.  A palette
+// index outside the palette (corrupt input only) yields a zero word and raises
+// kErrPaletteIndex instead of reading out of bounds.
+__device__ __forceinline__ uint32_t Rgb565(int y, int co, int cg) {
+  const int t = y - cg / 2;
+  const int g = cg + t;
+  const int b = (t - co) / 2;
+  const int r = b + co;
+  return ((static_cast<uint32_t>(r) << 11) | (static_cast<uint32_t>(g) << 5) |
+          static_cast<uint32_t>(b)) & 0xFFFFu;
+}
+
+__device__ __forceinline__ int SByte(uint32_t w, int q) {
+  return static_cast<int>(static_cast<int8_t>((w >> (8 * q)) & 0xFFu));
+}
+
+__global__ __launch_bounds__(256) void AssembleDxtKernel(KernelArgs a) {
+  const uint32_t total = StageTotal(a, kStageAssemble, 1);
+  for (uint32_t u = blockIdx.x; u < total; u += gridDim.x) {
+    const Located loc = Locate(a, kStageAssemble, u);
+    const ImageDesc d = LoadDesc(a, loc.img);
+    const uint32_t nv = d.nv;
+    const size_t b0 = static_cast<size_t>(loc.local) * kAssembleBlocksPerUnit + 4 * threadIdx.x;
+    uint32_t pw[6];
+#pragma unroll
+    for (int p = 0; p < 6; ++p)
+      pw[p] = *reinterpret_cast<const uint32_t *>(d.planes + static_cast<size_t>(p) * nv + b0);
+    const int4 ix = *reinterpret_cast<const int4 *>(d.indices + b0);
+    const uint32_t *pal = reinterpret_cast<const uint32_t *>(d.symbols + SymbolOffset(d, 2));
+    const uint32_t words = d.palette_bytes / 4;
+    const int idx[4] = {ix.x, ix.y, ix.z, ix.w};
+    uint32_t o[8];
+    bool bad = false;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const uint32_t ep1 = Rgb565(SByte(pw[0], q), SByte(pw[2], q), SByte(pw[3], q));
+      const uint32_t ep2 = Rgb565(SByte(pw[1], q), SByte(pw[4], q), SByte(pw[5], q));
+      const uint32_t pi = static_cast<uint32_t>(idx[q]);
+      uint32_t word = 0;
+      if (pi < words) word = pal[pi]; else bad = true;
+      o[2 * q] = ep1 | (ep2 << 16);
+      o[2 * q + 1] = word;
+    }
+    uint4 *dst = reinterpret_cast<uint4 *>(d.out + 8 * b0);
+    dst[0] = make_uint4(o[0], o[1], o[2], o[3]);
+    dst[1] = make_uint4(o[4], o[5], o[6], o[7]);
+    if (bad) atomicOr(a.err, kErrPaletteIndex);
+  }
+}
+
+__global__ __launch_bounds__(256) void AssembleRgbKernel(KernelArgs a) {
+  const uint32_t total = StageTotal(a, kStageAssemble, 1);
+  for (uint32_t u = blockIdx.x; u < total; u += gridDim.x) {
+    const Located loc = Locate(a, kStageAssemble, u);
+    const ImageDesc d = LoadDesc(a, loc.img);
+    const uint32_t nv = d.nv;
+    const size_t b0 = static_cast<size_t>(loc.local) * kAssembleBlocksPerUnit + 4 * threadIdx.x;
+    uint32_t pw[6];
+#pragma unroll
+    for (int p = 0; p < 6; ++p)
+      pw[p] = *reinterpret_cast<const uint32_t *>(d.planes + static_cast<size_t>(p) * nv + b0);
+    const int4 ix = *reinterpret_cast<const int4 *>(d.indices + b0);
+    const uint32_t *pal = reinterpret_cast<const uint32_t *>(d.symbols + SymbolOffset(d, 2));
+    const uint32_t words = d.palette_bytes / 4;
+    const int idx[4] = {ix.x, ix.y, ix.z, ix.w};
+    bool bad = false;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      int pal8[4][3];
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        const int y = SByte(pw[e], q), co = SByte(pw[2 + 2 * e], q), cg = SByte(pw[3 + 2 * e], q);
+        const int t = y - cg / 2;
+        const int g = cg + t;
+        const int b = (t - co) / 2;
+        const int r = b + co;
+        pal8[e][0] = static_cast<int>((static_cast<uint32_t>(r) << 3) | static_cast<uint32_t>(r >> 2));
+        pal8[e][1] = static_cast<int>((static_cast<uint32_t>(g) << 2) | static_cast<uint32_t>(g >> 4));
+        pal8[e][2] = static_cast<int>((static_cast<uint32_t>(b) << 3) | static_cast<uint32_t>(b >> 2));
+      }
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        pal8[2][c] = (2 * pal8[0][c] + pal8[1][c]) / 3;
+        pal8[3][c] = (pal8[0][c] + 2 * pal8[1][c]) / 3;
+      }
+      const uint32_t pi = static_cast<uint32_t>(idx[q]);
+      uint32_t word = 0;
+      if (pi < words) word = pal[pi]; else bad = true;
+      const size_t blk = b0 + q;
+      const uint32_t bxi = static_cast<uint32_t>(blk % d.bx), byi = static_cast<uint32_t>(blk / d.bx);
+#pragma unroll
+      for (int row = 0; row < 4; ++row) {
+        uint32_t bytes[12];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const int sel = (word >> (2 * (4 * row + c))) & 3;
+          bytes[3 * c + 0] = static_cast<uint32_t>(pal8[sel][0]) & 0xFFu;
+          bytes[3 * c + 1] = static_cast<uint32_t>(pal8[sel][1]) & 0xFFu;
+          bytes[3 * c + 2] = static_cast<uint32_t>(pal8[sel][2]) & 0xFFu;
+        }
+        uint32_t *dst = reinterpret_cast<uint32_t *>(
+            d.out + 3 * (static_cast<size_t>(d.width) * (4 * byi + row) + 4 * bxi));
+#pragma unroll
+        for (int w = 0; w < 3; ++w)
+          dst[w] = bytes[4 * w] | (bytes[4 * w + 1] << 8) | (bytes[4 * w + 2] << 16) |
+                   (bytes[4 * w + 3] << 24);
+      }
+    }
+    if (bad) atomicOr(a.err, kErrPaletteIndex);
+  }
+}
+
+template <uint32_t WAVES, uint32_t GPW>
+hipError_t LaunchAns(const KernelArgs &args, uint32_t grid, hipStream_t stream) {
+  hipLaunchKernelGGL((AnsDecodeKernel<WAVES, GPW>), dim3(grid), dim3(WAVES * 64), 0, stream, args);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t LaunchStage(uint32_t stage, const KernelArgs &args, uint32_t grid,
+                       const StageConfig &cfg, hipStream_t stream) {
+  if (grid == 0) return hipSuccess;
+  switch (stage) {
+    case kStageTables:
+      hipLaunchKernelGGL(BuildTableKernel, dim3(grid), dim3(256), 0, stream, args);
+      return hipGetLastError();
+    case kStageAns: {
+      const uint32_t w = cfg.ans_waves, g = cfg.ans_groups_per_wave;
+      if (g == 2 && w == 1) return LaunchAns<1, 2>(args, grid, stream);
+      if (g == 2 && w == 2) return LaunchAns<2, 2>(args, grid, stream);
+      if (g == 2 && w == 4) return LaunchAns<4, 2>(args, grid, stream);
+      if (g == 2 && w == 8) return LaunchAns<8, 2>(args, grid, stream);
+      if (g == 1 && w == 2) return LaunchAns<2, 1>(args, grid, stream);
+      if (g == 1 && w == 4) return LaunchAns<4, 1>(args, grid, stream);
+      if (g == 1 && w == 8) return LaunchAns<8, 1>(args, grid, stream);
+      return hipErrorInvalidValue;
+    }
+    case kStageIndices:
+      hipLaunchKernelGGL(DecodeIndicesKernel, dim3(grid), dim3(256), 0, stream, args);
+      return hipGetLastError();
+    case kStageWavelet:
+      hipLaunchKernelGGL(InvWaveletKernel, dim3(grid), dim3(64), 0, stream, args);
+      return hipGetLastError();
+    case kStageAssemble:
+      if (cfg.rgb)
+        hipLaunchKernelGGL(AssembleRgbKernel, dim3(grid), dim3(256), 0, stream, args);
+      else
+        hipLaunchKernelGGL(AssembleDxtKernel, dim3(grid), dim3(256), 0, stream, args);
+      return hipGetLastError();
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace gst

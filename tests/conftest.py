@@ -1,0 +1,64 @@
+"""Shared fixtures.  GPU tests carry @pytest.mark.gpu and run on an MI355X only."""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+GOLDEN_TEXTURES = ("ref_test1", "ref_c0", "ref_small", "ref_wide")
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) and the built libgst_amd.so")
+
+
+def load_golden(name):
+    with np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+def load_hist():
+    with open(os.path.join(GOLDEN, "ref_hist.json")) as f:
+        return json.load(f)
+
+
+class Device:
+    """torch-backed device memory and the current stream, for C-ABI calls."""
+
+    def __init__(self, torch):
+        self.torch = torch
+        self.dev = torch.device("cuda:0")
+
+    @property
+    def stream(self):
+        return self.torch.cuda.current_stream().cuda_stream
+
+    def empty(self, nbytes):
+        return self.torch.empty(max(int(nbytes), 1), dtype=self.torch.uint8, device=self.dev)
+
+    def upload(self, data):
+        host = self.torch.frombuffer(bytearray(bytes(data)), dtype=self.torch.uint8)
+        return host.to(self.dev)
+
+    def download(self, t, nbytes=None):
+        self.torch.cuda.synchronize()
+        arr = t.cpu().numpy()
+        return arr[: nbytes if nbytes is not None else len(arr)].tobytes()
+
+    def sync(self):
+        self.torch.cuda.synchronize()
+
+
+@pytest.fixture(scope="session")
+def device():
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import gst_amd
+    assert gst_amd.InitializeDecoder(0), "device 0 is not a usable gfx950"
+    return Device(torch)

@@ -1,0 +1,124 @@
+#!/usr/bin/env python3
+"""Writes the golden vectors under tests/golden/ from the reference's own code.
+
+Runs only where the reference sources are present (/root/reference): it builds
+oracle/_ref/ref_driver (oracle/Makefile target `ref`, the reference's encoder and
+CPU stage functions compiled from their sources plus a small driver) and records
+its outputs as data fixtures:
+
+  ref_test1.npz     the reference's own test image codec/test/test1.png encoded by
+                    the reference encoder (DXTImage + CompressDXT); `dxt` is
+                    PhysicalBlocks(), the value codec/test/codec_test.cpp:36-48
+                    requires of the decoder; `symbols` / `planes` are the reference
+                    CPU rANS decoder's and CPU inverse wavelet's outputs.
+  ref_c0.npz        the 512x512 seed-0 synthetic texture (noise 0.3) encoded the same way.
+  ref_small.npz     a 512x256 texture (smallest legal size, one index group).
+  ref_wide.npz      a 1024x512 texture (three index-scan stages).
+                    (These two keep only `gtc` and `dxt`.)
+  ref_ans_groups.npz  single-group rANS known answers from ans::EncodeInterleaved.
+  ref_hist.json     ans::GenerateHistogram outputs used by the table known answers
+                    of ans/ans_ocl_test.cpp:64-155.
+
+The synthetic RGB comes from this repository's generator (libgst_encode.so), so
+the same pixels can be regenerated anywhere; the encoded bytes are the reference's.
+"""
+import ctypes
+import json
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+REF = "/root/reference"
+DRIVER = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+
+
+def run(*args):
+    subprocess.run([DRIVER, *[str(a) for a in args]], check=True)
+
+
+def read(path):
+    with open(path, "rb") as f:
+        return f.read()
+
+
+def encode_case(name, rgb_path=None, png=None, w=None, h=None, meta=None, tmp=None,
+                stages=True):
+    gtc, dxt = os.path.join(tmp, name + ".gtc"), os.path.join(tmp, name + ".dxt")
+    sym, planes = os.path.join(tmp, name + ".sym"), os.path.join(tmp, name + ".planes")
+    if png:
+        run("encode_png", png, gtc, dxt)
+    else:
+        run("encode_rgb", rgb_path, w, h, gtc, dxt)
+    arrays = dict(gtc=np.frombuffer(read(gtc), np.uint8), dxt=np.frombuffer(read(dxt), np.uint8),
+                  meta=np.frombuffer(json.dumps(meta or {}).encode(), np.uint8))
+    if stages:
+        run("stages", gtc, sym, planes)
+        arrays["symbols"] = np.frombuffer(read(sym), np.uint8)
+        arrays["planes"] = np.frombuffer(read(planes), np.int8)
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **arrays)
+
+
+def synthetic_rgb(seed, w, h, noise_milli, path):
+    lib = ctypes.CDLL(os.path.join(ROOT, "gst_amd", "libgst_encode.so"))
+    buf = ctypes.create_string_buffer(w * h * 3)
+    lib.gste_generate_rgb(ctypes.c_uint64(seed), w, h, noise_milli, buf)
+    with open(path, "wb") as f:
+        f.write(buf.raw)
+
+
+def main():
+    if not os.path.isdir(REF):
+        sys.exit("reference sources not present; golden vectors are committed data")
+    subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
+    with tempfile.TemporaryDirectory() as tmp:
+        encode_case("ref_test1", png=os.path.join(REF, "codec", "test", "test1.png"),
+                    meta={"source": "codec/test/test1.png"}, tmp=tmp)
+        for name, seed, w, h in (("ref_c0", 0, 512, 512), ("ref_small", 3, 512, 256),
+                                 ("ref_wide", 7, 1024, 512)):
+            rgb = os.path.join(tmp, name + ".rgb")
+            synthetic_rgb(seed, w, h, 300, rgb)
+            encode_case(name, rgb_path=rgb, w=w, h=h, tmp=tmp, stages=(name == "ref_c0"),
+                        meta={"generator": "gste_generate_rgb", "seed": seed, "width": w,
+                              "height": h, "noise_milli": 300})
+
+        # Single-group rANS known answers.  The first count vector is the one of
+        # ans/ans_ocl_test.cpp DecodeSingleStream; the others cover a 256-symbol
+        # alphabet and a one-symbol-dominant alphabet.
+        cases = [
+            (1, [12, 14, 17, 1, 1, 2, 372]),
+            (2, [1] * 256),
+            (3, [5000] + [1] * 20),
+            (4, list(range(1, 200))),
+        ]
+        groups = {}
+        for seed, counts in cases:
+            path = os.path.join(tmp, "g%d.bin" % seed)
+            run("ans_group", seed, path, *counts)
+            raw = np.frombuffer(read(path), np.uint8)
+            freqs = raw[:512].view("<u2")
+            n = int(raw[512:516].view("<u4")[0])
+            groups["freq_%d" % seed] = freqs.copy()
+            groups["group_%d" % seed] = raw[516:516 + n].copy()
+            groups["symbols_%d" % seed] = raw[516 + n:516 + n + 8192].copy()
+        np.savez_compressed(os.path.join(HERE, "ref_ans_groups.npz"), **groups)
+
+    hist = {}
+    for key, counts in (("ocl_init", [3, 2, 1, 4, 3]),
+                        ("ocl_rebuild", [80, 300, 2, 14, 1, 1, 1, 20]),
+                        ("ocl_single", [12, 14, 17, 1, 1, 2, 372]),
+                        ("hist_kat", [1, 2, 3, 4, 5, 6, 7, 8, 9, 10])):
+        m = 256 if key == "hist_kat" else 2048
+        out = subprocess.run([DRIVER, "hist", str(m), *map(str, counts)], check=True,
+                             capture_output=True, text=True).stdout.split()
+        hist[key] = {"counts": counts, "M": m, "normalized": [int(v) for v in out]}
+    with open(os.path.join(HERE, "ref_hist.json"), "w") as f:
+        json.dump(hist, f, indent=1)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,122 @@
+"""Batched decode of textures of uneven size: one plan, persistent launches, and the
+captured hipGraph replayed for different bound batches."""
+import numpy as np
+import pytest
+
+from conftest import load_golden
+
+import gst_amd
+from gst_amd import encode
+from oracle import gst_oracle as go
+
+pytestmark = pytest.mark.gpu
+
+MIXED = [(301, 512, 512), (302, 1024, 1024), (303, 512, 1024), (304, 2048, 2048), (305, 768, 512),
+         (306, 512, 256), (307, 1024, 512)]
+
+
+def _bind(device, batch, files):
+    hdrs = [gst_amd.ParseHeader(f) for f in files]
+    d_files = [device.upload(f) for f in files]
+    d_outs = [device.empty(h.dxt_bytes) for h in hdrs]
+    batch.bind(hdrs, [t.data_ptr() for t in d_files], [t.data_ptr() for t in d_outs], device.stream)
+    return hdrs, d_files, d_outs
+
+
+def _capacity(files):
+    hdrs = [gst_amd.ParseHeader(f) for f in files]
+    return sum(h.num_blocks for h in hdrs), sum(h.palette_bytes for h in hdrs)
+
+
+def test_uneven_batch_matches_oracle(device):
+    files, dxts = encode.make_textures(MIXED)
+    files += [load_golden("ref_test1")["gtc"].tobytes()]
+    dxts += [load_golden("ref_test1")["dxt"].tobytes()]
+    blocks, pal = _capacity(files)
+    b = gst_amd.Batch(len(files), blocks, pal)
+    try:
+        hdrs, d_files, d_outs = _bind(device, b, files)
+        b.decode(device.stream)
+        for f, d, out in zip(files, dxts, d_outs):
+            rc, expected, _ = go.decode_dxt(f, nthreads=8)
+            assert rc == go.OK and expected == d
+            assert device.download(out) == expected
+        assert b.error_bits(device.stream) == 0
+    finally:
+        b.close()
+
+
+@pytest.mark.parametrize("gpw,waves", [(2, 4), (1, 4), (2, 8), (2, 1)])
+def test_rans_launch_shapes_agree(device, gpw, waves):
+    files, dxts = encode.make_textures(MIXED[:4])
+    blocks, pal = _capacity(files)
+    b = gst_amd.Batch(len(files), blocks, pal, groups_per_wave=gpw, waves_per_group=waves,
+                      grids={"ans_decode": 64, "inv_wavelet": 128, "decode_indices": 16, "assemble": 32})
+    try:
+        _, _, d_outs = _bind(device, b, files)
+        b.decode(device.stream)
+        for d, out in zip(dxts, d_outs):
+            assert device.download(out) == d
+    finally:
+        b.close()
+
+
+def test_graph_replays_match_eager_and_follow_rebinds(device):
+    torch = device.torch
+    files_a, dxts_a = encode.make_textures(MIXED[:5])
+    files_b, dxts_b = encode.make_textures([(401, 1024, 1024), (402, 512, 256), (403, 512, 512)])
+    blocks = max(_capacity(files_a)[0], _capacity(files_b)[0])
+    pal = max(_capacity(files_a)[1], _capacity(files_b)[1])
+    b = gst_amd.Batch(8, blocks, pal)
+    try:
+        _, keep_a, outs_a = _bind(device, b, files_a)
+        b.decode(device.stream)
+        eager = [device.download(o) for o in outs_a]
+        assert eager == dxts_a
+        b.capture_graph(device.stream)
+        for rep in range(3):
+            for o in outs_a:
+                o.zero_()
+            scratch = torch.randn(1 << 20, device=device.dev)  # eager work between replays
+            scratch.mul_(2.0)
+            b.launch_graph(device.stream)
+            device.sync()
+            assert [device.download(o) for o in outs_a] == eager, "replay %d" % rep
+        # Rebind a different batch to the same plan; the captured graph follows it.
+        _, keep_b, outs_b = _bind(device, b, files_b)
+        b.launch_graph(device.stream)
+        assert [device.download(o) for o in outs_b] == dxts_b
+        assert b.error_bits(device.stream) == 0
+    finally:
+        b.close()
+
+
+def test_decode_timed_reports_every_stage(device):
+    files, dxts = encode.make_textures(MIXED[:3])
+    blocks, pal = _capacity(files)
+    b = gst_amd.Batch(len(files), blocks, pal)
+    try:
+        _, _, outs = _bind(device, b, files)
+        ms = b.decode_timed(device.stream)
+        assert set(ms) == set(gst_amd.STAGES) and all(v > 0 for v in ms.values())
+        assert [device.download(o) for o in outs] == dxts
+    finally:
+        b.close()
+
+
+def test_bind_rejects_batches_over_capacity(device):
+    files, _ = encode.make_textures(MIXED[:2])
+    blocks, pal = _capacity(files)
+    b = gst_amd.Batch(1, blocks, pal)
+    try:
+        with pytest.raises(gst_amd.GstError):
+            _bind(device, b, files)
+    finally:
+        b.close()
+    b = gst_amd.Batch(4, blocks - 8192, pal)
+    try:
+        with pytest.raises(gst_amd.GstError) as e:
+            _bind(device, b, files)
+        assert e.value.code == -4
+    finally:
+        b.close()

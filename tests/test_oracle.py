@@ -1,0 +1,257 @@
+"""Pins the CPU oracle (oracle/gst_oracle.c) against the reference's own outputs,
+known answers and closed forms.  CPU only."""
+import json
+import struct
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN_TEXTURES, load_golden, load_hist
+from oracle import gst_oracle as go
+
+
+# ---- end to end: codec/test/codec_test.cpp:36-48 ----------------------------------
+
+@pytest.mark.parametrize("name", GOLDEN_TEXTURES)
+def test_reference_encoded_files_decode_to_physical_blocks(name):
+    g = load_golden(name)
+    rc, out, _ = go.decode_dxt(g["gtc"].tobytes(), nthreads=4)
+    assert rc == go.OK
+    assert out == g["dxt"].tobytes()
+
+
+@pytest.mark.parametrize("name", ["ref_test1", "ref_c0"])
+def test_stage_outputs_match_reference_cpu_functions(name):
+    """ANS symbols vs ans::DecodeInterleaved, planes vs InverseWavelet2D."""
+    g = load_golden(name)
+    rc, _, st = go.decode_dxt(g["gtc"].tobytes(), stages=True)
+    assert rc == go.OK
+    np.testing.assert_array_equal(st["symbols"], g["symbols"])
+    np.testing.assert_array_equal(st["planes"], g["planes"])
+
+
+def test_single_thread_equals_threaded():
+    g = load_golden("ref_wide")
+    a = go.decode_dxt(g["gtc"].tobytes(), nthreads=1)[1]
+    b = go.decode_dxt(g["gtc"].tobytes(), nthreads=8)[1]
+    assert a == b == g["dxt"].tobytes()
+
+
+# ---- build_table: ans/build_table.cl, ans/ans_ocl_test.cpp:64-155 ------------------
+
+def closed_form_table(freqs):
+    freqs = np.asarray(freqs, np.int64)
+    cum = np.concatenate([[0], np.cumsum(freqs)[:-1]])
+    sym = np.repeat(np.arange(256), freqs)
+    assert len(sym) == 2048
+    return freqs[sym], cum[sym], sym
+
+
+def test_search_path_depends_only_on_target_symbol():
+    """build_table.cl:61-75 with comparisons resolved by the target symbol t: for a
+    table summing to 2048, id < cum[x] iff x > t and cum[x+1] <= id iff x < t, so
+    simulating the 11 steps per t covers every table."""
+    for t in range(256):
+        low, high = 0, 255
+        x = (high + low) // 2
+        for _ in range(11):
+            too_high = 1 if x > t else 0
+            too_low = 1 if (x < 255 and x < t) else 0
+            low = max(low + 1, x) if too_low else low
+            high = (min(high - 1, x) if high >= 1 else x) if too_high else high
+            x = (high + low) // 2
+        assert x == t
+
+
+def _padded(vals):
+    f = np.zeros(256, np.uint16)
+    f[: len(vals)] = vals
+    return f
+
+
+@pytest.mark.parametrize("key", ["ocl_init", "ocl_rebuild", "ocl_single"])
+def test_table_known_answers(key):
+    """ans_ocl_test.cpp Initialization / TableRebuilding: expected symbol, frequency
+    and cumulative arrays built from the reference's GenerateHistogram output."""
+    h = load_hist()[key]
+    norm = h["normalized"]
+    assert sum(norm) == 2048
+    expected_sym, expected_freq, expected_cum = [], [], []
+    s = 0
+    for i, f in enumerate(norm):
+        expected_sym += [i] * f
+        expected_freq += [f] * f
+        expected_cum += [s] * f
+        s += f
+    freq, cum, sym = go.build_table(_padded(norm))
+    np.testing.assert_array_equal(sym, expected_sym)
+    np.testing.assert_array_equal(freq, expected_freq)
+    np.testing.assert_array_equal(cum, expected_cum)
+
+
+def test_histogram_known_answer_recorded():
+    """histogram_test.cpp ProperlyDistributesFreqs via the compiled reference."""
+    assert load_hist()["hist_kat"]["normalized"] == [5, 9, 14, 19, 23, 28, 33, 37, 42, 46]
+
+
+def test_table_closed_form_random_and_edge_tables():
+    rng = np.random.default_rng(7)
+    tables = [_padded([2048]), np.r_[np.zeros(255, np.uint16), [2048]].astype(np.uint16),
+              np.full(256, 8, np.uint16)]
+    for _ in range(40):
+        k = int(rng.integers(1, 257))
+        idx = rng.choice(256, size=k, replace=False)
+        cuts = np.sort(rng.choice(np.arange(1, 2048), size=k - 1, replace=False)) if k > 1 else []
+        parts = np.diff(np.r_[0, cuts, 2048])
+        f = np.zeros(256, np.uint16)
+        f[idx] = parts
+        tables.append(f)
+    for f in tables:
+        ef, ec, es = closed_form_table(f)
+        freq, cum, sym = go.build_table(f)
+        np.testing.assert_array_equal(freq, ef)
+        np.testing.assert_array_equal(cum, ec)
+        np.testing.assert_array_equal(sym, es)
+
+
+# ---- ans_decode: ans/ans_decode.cl ---------------------------------------------------
+
+def _wrap_group(group):
+    """A one-group stream as EncodeBytes lays it out (codec/entropy.cpp:213-231)."""
+    g = bytes(group)
+    pad = 2 if len(g) % 4 else 0
+    body = b"\0" * pad + g
+    return struct.pack("<I", 4 + len(body)) + body, pad
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3, 4])
+def test_ans_group_known_answers(seed):
+    z = load_golden("ref_ans_groups")
+    freqs, group, expected = z["freq_%d" % seed], z["group_%d" % seed], z["symbols_%d" % seed]
+    stream, pad = _wrap_group(group)
+    rc, out, states, word = go.ans_decode_group(freqs, stream, 1, 0)
+    assert rc == go.OK
+    np.testing.assert_array_equal(out, expected)
+    # Invariants of a complete decode: every state is back at k*M, the encoder's
+    # initial state (ans/encode.cpp:47), and the pointer is at the group's first word.
+    assert np.all(states == 32768)
+    assert 2 * word == 4 + pad
+
+
+def test_ans_group_rejects_offsets_outside_stream():
+    z = load_golden("ref_ans_groups")
+    stream, _ = _wrap_group(z["group_1"])
+    bad = struct.pack("<I", len(stream) + 8) + stream[4:]
+    assert go.ans_decode_group(z["freq_1"], bad, 1, 0)[0] == go.ERR_STREAM
+    # A truncated group makes the renormalisation reads run past the group start.
+    trunc = struct.pack("<I", 4 + 200) + stream[-200:]
+    assert go.ans_decode_group(z["freq_1"], trunc, 1, 0)[0] == go.ERR_STREAM
+
+
+# ---- decode_indices: codec/decode_indices.cl, codec/decoder.cpp:304-393 --------------
+
+def staged_scan_exact(nv):
+    """The reference's staged scan covers every element iff each stage with at least
+    128 entries has a multiple of 128 of them (else the last entry misses a carry)."""
+    s = 1
+    while (nv >> (7 * s)) > 0:
+        n = nv >> (7 * s)
+        if n >= 128 and n % 128:
+            return False
+        s += 1
+    return True
+
+
+@pytest.mark.parametrize("k", list(range(1, 65)) + [96, 127, 128, 384])
+def test_staged_reference_scan_against_prefix_sum(k):
+    nv = 8192 * k
+    rng = np.random.default_rng(k)
+    d = rng.integers(0, 256, nv, dtype=np.uint8)
+    ref = go.decode_indices(d)
+    np.testing.assert_array_equal(ref, np.cumsum(d.astype(np.int64) - 128).astype(np.int32))
+    for race, garbage in ((0, 0), (1, 12345)):
+        staged = go.decode_indices_staged(d, race, garbage)
+        np.testing.assert_array_equal(staged[:-1], ref[:-1])
+        if staged_scan_exact(nv):
+            assert staged[-1] == ref[-1]
+        else:
+            # Reference defect: the last index misses earlier superchunks' carry and
+            # (k odd) two work-items race to store it; the oracle keeps the prefix sum.
+            assert staged[-1] != ref[-1]
+
+
+def test_benchmark_sizes_have_exact_reference_scan():
+    for w, h in ((512, 512), (512, 256), (1024, 1024), (512, 1024), (1024, 512), (2048, 2048),
+                 (4096, 4096)):
+        assert staged_scan_exact(w * h // 16)
+
+
+# ---- inv_wavelet: codec/inverse_wavelet.cl, codec/test/wavelet_test.cpp:129-158 ------
+
+def test_inverse_wavelet_4x4_known_answer():
+    xs = np.array([63, 64, 0, -1, 66, 60, 6, 9, 0, 2, -2, -2, 7, -18, 16, 36], np.int32).reshape(4, 4)
+    expected = np.array([63, 63, 63, 63, 63, 63, 64, 63, 63, 65, 62, 64, 62, 65, 31, 69],
+                        np.int32).reshape(4, 4)
+    tile = np.zeros((32, 32), np.int32)
+    tile[:4, :4] = xs
+    out = go.inv_wavelet_level(tile.ravel(), 4).reshape(32, 32)
+    np.testing.assert_array_equal(out[:4, :4], expected)
+    assert np.all(out[4:, :] == 0) and np.all(out[:, 4:] == 0)
+
+
+def test_inverse_wavelet_tile_is_level_composition():
+    rng = np.random.default_rng(3)
+    for _ in range(20):
+        tile = rng.integers(0, 256, 1024, dtype=np.uint8)
+        t = tile.astype(np.int32) - 128
+        for ln in (2, 4, 8, 16, 32):
+            t = go.inv_wavelet_level(t, ln)
+        np.testing.assert_array_equal(go.inv_wavelet_tile(tile), t.astype(np.int8))
+
+
+# ---- assemble: codec/assemble.cl, codec/image_processing.cpp:10-46 -------------------
+
+def test_ycocg_to_rgb565_inverts_encoder_transform_exhaustively():
+    """All 65536 RGB565 values through the encoder's rgb565_to_ycocg667
+    (codec/image_processing.cpp:10-27, restated with C truncating division)."""
+    for v in range(65536):
+        r, g, b = v >> 11, (v >> 5) & 63, v & 31
+        co = r - b
+        t = r + b + (b >> 4)
+        cg = g - t
+        y = t + int(cg / 2)
+        assert 0 <= y <= 63 and -31 <= co <= 31 and -63 <= cg <= 63
+        assert go.ycocg_to_rgb565(y, co, cg) == v
+
+
+def test_rgb565_pack_keeps_low_bits_for_out_of_range_channels():
+    # (u16)(R << 11 | G << 5 | B) on int32 (codec/assemble.cl:324-338)
+    for y, co, cg in ((0, 31, -63), (63, -31, 63), (-5, 17, 100), (127, -128, -128)):
+        t = y - int(cg / 2)
+        g = cg + t
+        b = int((t - co) / 2)
+        r = b + co
+        assert go.ycocg_to_rgb565(y, co, cg) == ((r << 11) | (g << 5) | b) & 0xFFFF
+
+
+# ---- headers --------------------------------------------------------------------------
+
+def test_header_rules():
+    g = load_golden("ref_c0")["gtc"].tobytes()
+    assert go.parse_header(g)[0] == go.OK
+    for field, value in ((0, 500), (1, 0), (2, 100), (2, 0), (3, 6)):
+        bad = bytearray(g)
+        struct.pack_into("<I", bad, 4 * field, value)
+        assert go.parse_header(bytes(bad))[0] == go.ERR_HEADER
+    assert go.parse_header(g[:-4])[0] == go.ERR_HEADER
+    # 768x512 is legal (w*h = 3 * 131072); 640x512 is not.
+    hdr = bytearray(g)
+    struct.pack_into("<II", hdr, 0, 640, 512)
+    assert go.parse_header(bytes(hdr))[0] == go.ERR_HEADER
+
+
+def test_frequency_tables_must_sum_to_table_size():
+    g = bytearray(load_golden("ref_c0")["gtc"].tobytes())
+    f = struct.unpack_from("<H", g, 28)[0]
+    struct.pack_into("<H", g, 28, f + 1)
+    assert go.decode_dxt(bytes(g))[0] == go.ERR_FREQ
