@@ -67,6 +67,9 @@ class Gpu:
         import torch
         self.torch = torch
         self.dev = torch.device("cuda:0")
+        # A created stream: graph capture is not possible on the null stream.
+        self.main = torch.cuda.Stream(device=self.dev)
+        torch.cuda.set_stream(self.main)
 
     @property
     def stream(self):

@@ -63,6 +63,15 @@ def build_encoder():
                                          "-o", o])
 
 
+def build_cpp_tests():
+    """tests/cpp/codec_test: the reference's end-to-end test against the C++ API."""
+    src = os.path.join(ROOT, "tests", "cpp", "codec_test.cpp")
+    out = os.path.join(ROOT, "tests", "cpp", "codec_test")
+    return _atomic_build(out, lambda o: [HIPCC, "--offload-arch=gfx950", "-O2", "-std=c++17",
+                                         "-Wno-unused-command-line-argument", "-I" + INCLUDE, src,
+                                         "-L" + HERE, "-lgst_amd", "-Wl,-rpath," + HERE, "-o", o])
+
+
 def build_oracle():
     """The checker libraries (not part of the product)."""
     oracle = os.path.join(ROOT, "oracle")
@@ -74,6 +83,7 @@ def build_oracle():
 def build_all():
     build_decoder()
     build_encoder()
+    build_cpp_tests()
     build_oracle()
 
 

@@ -187,12 +187,16 @@ __device__ __forceinline__ void DecodeHalfGroups(const uint2 *table, __amdgpu_bu
   }
 }
 
-__device__ __forceinline__ uint32_t Uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+// readfirstlane returns a signed int: every use converts through uint32_t so no
+// 64-bit widening can sign-extend a value with bit 31 set.
+__device__ __forceinline__ uint32_t Uniform(uint32_t v) {
+  return static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(v)));
+}
 
 __device__ __forceinline__ const uint8_t *Uniform(const uint8_t *p) {
   const uint64_t v = reinterpret_cast<uint64_t>(p);
-  const uint64_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v));
-  const uint64_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v >> 32));
+  const uint64_t lo = Uniform(static_cast<uint32_t>(v));
+  const uint64_t hi = Uniform(static_cast<uint32_t>(v >> 32));
   return reinterpret_cast<const uint8_t *>(lo | (hi << 32));
 }
 

@@ -33,6 +33,9 @@ class Device:
     def __init__(self, torch):
         self.torch = torch
         self.dev = torch.device("cuda:0")
+        # A created stream (graph capture is not possible on the null stream).
+        self.side = torch.cuda.Stream(device=self.dev)
+        torch.cuda.set_stream(self.side)
 
     @property
     def stream(self):

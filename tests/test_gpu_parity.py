@@ -1,5 +1,6 @@
 """Parity of the gfx950 decoder (through the C ABI) with the oracle and the reference's
 golden vectors.  Bit-exact everywhere: all stages are integer work."""
+import os
 import struct
 
 import numpy as np
@@ -210,3 +211,17 @@ def test_garbled_renormalisation_words_stay_in_bounds(device):
         assert len(out) == h.dxt_bytes
     except gst_amd.GstError as e:
         assert e.code == -3
+
+
+@pytest.mark.parametrize("name", ["ref_test1", "ref_wide"])
+def test_cpp_codec_test_against_reference_blocks(device, name, tmp_path):
+    """tests/cpp/codec_test.cpp: codec/test/codec_test.cpp:36-48 through the C++ API."""
+    import subprocess
+    from conftest import ROOT
+    g = load_golden(name)
+    (tmp_path / "f.gtc").write_bytes(g["gtc"].tobytes())
+    (tmp_path / "f.dxt").write_bytes(g["dxt"].tobytes())
+    exe = os.path.join(ROOT, "tests", "cpp", "codec_test")
+    proc = subprocess.run([exe, str(tmp_path / "f.gtc"), str(tmp_path / "f.dxt")], capture_output=True,
+                          text=True, timeout=60)
+    assert proc.returncode == 0, proc.stdout + proc.stderr
