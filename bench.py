@@ -200,15 +200,27 @@ def main():
     ap.add_argument("--images", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the per-image comparison")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a device-to-device copy of 512 MiB (5x), the reference kernel for "
+                         "scaling FETCH_SIZE / WRITE_SIZE counters to bytes")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (--gpus %d requested)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
 
+    gpu = Gpu()
+    if args.copy_calibration:
+        n = 512 << 20
+        src = gpu.torch.ones(n, dtype=gpu.torch.uint8, device=gpu.dev)
+        dst = gpu.empty(n)
+        ms = gpu.timed(lambda: dst.copy_(src), 5)
+        print(json.dumps({"copy_bytes_each_way": n, "ms": round(ms, 4),
+                          "GBs_read_plus_write": round(2 * n / (ms * 1e-3) / 1e9, 1)}), flush=True)
+        return
+
     import gst_amd
     from gst_amd import encode
-    gpu = Gpu()
     if not gst_amd.InitializeDecoder(0):
         raise SystemExit("cuda:0 is not a usable gfx950")
 

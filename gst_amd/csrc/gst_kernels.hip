@@ -130,22 +130,71 @@ __global__ __launch_bounds__(256) void BuildTableKernel(KernelArgs a) {
 // (atomic_or / popcount / atomic_and between barriers) becomes one ballot: a lane
 // that needs a word reads u16[p_half - total_in_half + rank_in_half], the rank
 // coming from mbcnt over the 64-bit ballot, and each half-wave's pointer (a scalar)
-// moves down by its total.  All stream reads go through a buffer resource bounded
-// by the stream length, so corrupt offsets read zeros instead of faulting; a
-// pointer that ends below the group's first byte raises kErrStreamOffsets.  Each
-// lane packs 16 symbols into four registers and writes them with one 16-byte
-// store, in the reference's output order (lane j, step i of the group at
-// j*256 + 255 - i).  Index-stream groups also emit the sum of (symbol - 128), the
-// per-group total the index scan starts from.
-template <bool kSum>
-__device__ __forceinline__ void DecodeHalfGroups(const uint2 *table, __amdgpu_buffer_rsrc_t rs,
-                                                 uint32_t lane, bool active, uint32_t x,
-                                                 uint32_t pA, uint32_t pB, uint32_t loA,
+// moves down by its total.
+//
+// The words are read from an LDS ring of two 512-word chunks per half-wave, so the
+// serial chain of a step only touches LDS.  Chunk c covers stream words
+// [512c, 512c + 512) and lives at ring slot c & 1.  With `top` the chunk holding the
+// half's pointer - 1, chunks top and top-1 are resident and chunk top-2 is in flight
+// in registers.  A step moves the pointer by at most 32 words, so 16 steps read
+// only chunks top and top-1; every 16 steps, if the pointer has dropped below
+// 512*top, chunk top-2 is written over top's slot and chunk top-3 is requested,
+// which then has at least 16 steps to arrive.  Stream loads go through a buffer resource bounded by the stream
+// length (corrupt offsets read zeros, never fault); a pointer that ends below the
+// group's first byte raises kErrStreamOffsets.  Each lane packs 16 symbols into four
+// registers and writes them with one 16-byte store, in the reference's output order
+// (lane j, step i of the group at j*256 + 255 - i).  Index-stream groups also emit
+// the sum of (symbol - 128), the per-group total the index scan starts from.
+constexpr uint32_t kChunkWords = 512;
+constexpr uint32_t kRingWords = 2 * kChunkWords;
+
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// Lane j's 32 bytes of chunk c (words 16j .. 16j+15); chunks below the stream
+// start wrap to huge offsets and read zeros.
+__device__ __forceinline__ void LoadChunk(__amdgpu_buffer_rsrc_t rs, uint32_t c, uint32_t j, u32x4 &a,
+                                          u32x4 &b) {
+  const int off = static_cast<int>((c * kChunkWords + 16 * j) * 2);
+  a = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0);
+  b = __builtin_amdgcn_raw_buffer_load_b128(rs, off + 16, 0, 0);
+}
+
+__device__ __forceinline__ void StoreChunk(uint16_t *ring, uint32_t c, uint32_t j, const u32x4 &a,
+                                           const u32x4 &b) {
+  u32x4 *dst = reinterpret_cast<u32x4 *>(ring + ((c * kChunkWords + 16 * j) & (kRingWords - 1)));
+  dst[0] = a;
+  dst[1] = b;
+}
+
+template <bool kSum, uint32_t GPW>
+__device__ __forceinline__ void DecodeHalfGroups(const uint2 *table, uint16_t *ring_half,
+                                                 __amdgpu_buffer_rsrc_t rs, uint32_t lane, bool active,
+                                                 uint32_t x, uint32_t pA, uint32_t pB, uint32_t loA,
                                                  uint32_t loB, uint8_t *outp, int32_t *group_sum,
                                                  uint32_t *err) {
+  const uint32_t j = lane & 31;
   uint32_t upper = lane < 32 ? 0u : 0xFFFFFFFFu;
   asm volatile("" : "+v"(upper));  // keep the half-wave mask an opaque VGPR (one v_and below)
   const uint64_t active_mask = __builtin_amdgcn_ballot_w64(active);
+  // Ring prologue: chunks top and top-1 resident, top-2 in flight.
+  uint32_t topA = pA ? (pA - 1) / kChunkWords : 0;
+  uint32_t topB = pB ? (pB - 1) / kChunkWords : 0;
+  const uint32_t top = lane < 32 ? topA : topB;
+  {
+    u32x4 a0, a1, b0, b1;
+    LoadChunk(rs, top, j, a0, a1);
+    LoadChunk(rs, top - 1, j, b0, b1);
+    if (GPW == 2 || lane < 32) {  // one group per wave: the idle upper half shares the ring
+      StoreChunk(ring_half, top, j, a0, a1);
+      StoreChunk(ring_half, top - 1, j, b0, b1);
+    }
+  }
+  u32x4 pend0, pend1;
+  LoadChunk(rs, top - 2, j, pend0, pend1);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
   int32_t sum = 0;
   for (uint32_t blk = 0; blk < kSymsPerLane / 16; ++blk) {
     uint32_t acc0 = 0, acc1 = 0, acc2 = 0, acc3 = 0;
@@ -159,31 +208,47 @@ __device__ __forceinline__ void DecodeHalfGroups(const uint2 *table, __amdgpu_bu
       pA -= tlo;
       pB -= thi;
       if (active && x < kStateL) {
-        // Byte offset of u16[p_half + rank]: mbcnt over both halves counts the lower
-        // half's bits for upper lanes, so upper lanes start from pB - tlo - pA.
-        const uint32_t t = upper & (pB - tlo - pA);
-        const uint32_t widx = __builtin_amdgcn_mbcnt_hi(mhi, __builtin_amdgcn_mbcnt_lo(mlo, t));
-        const uint32_t w = __builtin_amdgcn_raw_buffer_load_b16(rs, static_cast<int>(widx * 2 + pA * 2), 0, 0);
-        x = (x << 16) | w;
+        // Word index p_half + rank: mbcnt over both halves counts the lower half's
+        // bits for upper lanes, so upper lanes start from pB - tlo.
+        const uint32_t b = pA + (upper & (pB - tlo - pA));
+        const uint32_t widx = __builtin_amdgcn_mbcnt_hi(mhi, __builtin_amdgcn_mbcnt_lo(mlo, b));
+        x = (x << 16) | ring_half[widx & (kRingWords - 1)];
       }
-      const uint32_t sym = e.x >> 24;
+      // Symbol (byte 3 of e.x) into byte `pos` of the 16-byte chunk: v_perm selector
+      // codes 4-7 take bytes of the first operand, 0-3 bytes of the second.
       const uint32_t pos = 15 - s;
-      const uint32_t bits = sym << (8 * (pos & 3));
-      if ((pos >> 2) == 0) acc0 |= bits;
-      else if ((pos >> 2) == 1) acc1 |= bits;
-      else if ((pos >> 2) == 2) acc2 |= bits;
-      else acc3 |= bits;
-      if (kSum) sum += static_cast<int32_t>(sym) - 128;
+      const uint32_t sel = (0x03020100u & ~(0xFFu << (8 * (pos & 3)))) | (7u << (8 * (pos & 3)));
+      if ((pos >> 2) == 0) acc0 = __builtin_amdgcn_perm(e.x, acc0, sel);
+      else if ((pos >> 2) == 1) acc1 = __builtin_amdgcn_perm(e.x, acc1, sel);
+      else if ((pos >> 2) == 2) acc2 = __builtin_amdgcn_perm(e.x, acc2, sel);
+      else acc3 = __builtin_amdgcn_perm(e.x, acc3, sel);
+      if (kSum) sum += static_cast<int32_t>(e.x >> 24) - 128;
     }
     if (active)
       *reinterpret_cast<uint4 *>(outp + (kSymsPerLane - 16) - 16 * blk) =
           make_uint4(acc0, acc1, acc2, acc3);
+    // Ring refill at the 16-step boundary: the pointer moved at most 512 words, so it
+    // crossed at most one chunk boundary, and the next 16 steps read only chunks
+    // top and top-1 (top = chunk of pointer - 1).
+    const bool crossA = pA < topA * kChunkWords, crossB = pB < topB * kChunkWords;
+    if (crossA || crossB) {
+      if (lane < 32 ? crossA : crossB) {
+        const uint32_t t = lane < 32 ? topA : topB;
+        StoreChunk(ring_half, t - 2, j, pend0, pend1);
+        LoadChunk(rs, t - 3, j, pend0, pend1);
+      }
+      topA -= crossA ? 1u : 0u;
+      topB -= crossB ? 1u : 0u;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
   }
   const bool under = lane < 32 ? (pA < loA) : (pB < loB);
-  if (active && under && (lane & 31) == 0) atomicOr(err, kErrStreamOffsets);
+  if (active && under && j == 0) atomicOr(err, kErrStreamOffsets);
   if (kSum) {
     sum = WaveSumWidth32(sum);
-    if (active && (lane & 31) == 0) *group_sum = sum;
+    if (active && j == 0) *group_sum = sum;
   }
 }
 
@@ -203,6 +268,7 @@ __device__ __forceinline__ const uint8_t *Uniform(const uint8_t *p) {
 template <uint32_t WAVES, uint32_t GPW>
 __global__ __launch_bounds__(WAVES * 64) void AnsDecodeKernel(KernelArgs a) {
   __shared__ uint2 table[kTableSize];
+  __shared__ uint16_t ring[WAVES][2][kRingWords];
   constexpr uint32_t kGroupsPerUnit = WAVES * GPW;
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint32_t half = GPW == 2 ? (lane >> 5) : 0;
@@ -258,11 +324,13 @@ __global__ __launch_bounds__(WAVES * 64) void AnsDecodeKernel(KernelArgs a) {
     const uint32_t pB = GPW == 2 ? __builtin_amdgcn_readlane(p, 32) : 0u;
     const uint32_t loB = GPW == 2 ? __builtin_amdgcn_readlane(plo, 32) : 0u;
     uint8_t *outp = d.symbols + SymbolOffset(d, k) + static_cast<size_t>(g) * kGroupSyms + j * kSymsPerLane;
+    uint16_t *ring_half = ring[wave][half];
     if (k == 3)
-      DecodeHalfGroups<true>(table, rs, lane, active, x, pA, pB, loA, loB, outp,
+      DecodeHalfGroups<true, GPW>(table, ring_half, rs, lane, active, x, pA, pB, loA, loB, outp,
                              d.group_sums + g, a.err);
     else
-      DecodeHalfGroups<false>(table, rs, lane, active, x, pA, pB, loA, loB, outp, nullptr, a.err);
+      DecodeHalfGroups<false, GPW>(table, ring_half, rs, lane, active, x, pA, pB, loA, loB, outp, nullptr,
+                              a.err);
   }
 }
 
