@@ -192,6 +192,73 @@ def run_cpu_baseline(files, min_seconds=10.0, max_passes=50):
     return float(np.median(rates)), threads, len(rates), t_total
 
 
+SERVED_MIX = [((512, 512), 0.50, 16), ((1024, 1024), 0.30, 12), ((2048, 2048), 0.15, 6),
+              ((4096, 4096), 0.05, 2)]
+
+
+def run_served(qps_list, duration_s, budget_gib=32, closed_loop=64):
+    """Poisson arrivals of the size mix through the continuous-batching server and
+    through the plain per-request server; latency is arrival -> output in HBM."""
+    from gst_amd import encode
+    from gst_amd.serve import Server
+    specs, weights = [], []
+    seed = 5000
+    for (w, h), p, count in SERVED_MIX:
+        for _ in range(count):
+            specs.append((seed, w, h))
+            weights.append(p / count)
+            seed += 1
+    pool, _ = encode.make_textures(specs, want_dxt=False, nthreads=min(16, os.cpu_count() or 1))
+    out = {"pool": "%d pre-encoded textures, size mix {512^2: 50%%, 1024^2: 30%%, 2048^2: 15%%, 4096^2: 5%%}"
+                   % len(pool), "hbm_budget_GiB": budget_gib, "duration_s_per_point": duration_s}
+
+    def summarize(lat_dev, lat_host, n, wall, st):
+        busy = st["device_busy_ms"] * 1e-3
+        return {
+            "requests": int(n),
+            "p50_us": round(float(np.percentile(lat_dev, 50)), 1),
+            "p99_us": round(float(np.percentile(lat_dev, 99)), 1),
+            "p50_with_d2h_us": round(float(np.percentile(lat_host, 50)), 1),
+            "p99_with_d2h_us": round(float(np.percentile(lat_host, 99)), 1),
+            "Mtexel_s": round(st["texels"] / wall / 1e6, 1),
+            "Mtexel_s_busy": round(st["texels"] / busy / 1e6, 1) if busy > 0 else None,
+            "busy_frac": round(busy / wall, 4),
+            "B_pipe_roofline_busy": round(st["algorithmic_bytes"] / busy / (HBM_PEAK_GBS * 1e9), 4)
+            if busy > 0 else None,
+            "mean_batch_images": round(st["mean_batch_images"], 2),
+            "max_batch_images": int(st["max_batch_images"]),
+        }
+
+    budget = budget_gib << 30
+    for mode in ("batched", "plain"):
+        res = {}
+        for qps in qps_list:
+            with Server(budget, plain=(mode == "plain"), copy_to_host=True, auto_release=True) as s:
+                s.run_load(pool[:4], qps=200.0, duration_s=0.2, seed=1)  # warm-up (graphs, pools)
+                base = s.stats()
+                lat_dev, lat_host, n, wall = s.run_load(pool, weights, qps=qps, duration_s=duration_s,
+                                                        seed=int(qps))
+                st = s.stats()
+                for k in ("texels", "algorithmic_bytes"):
+                    st[k] -= base[k]
+                st["device_busy_ms"] -= base["device_busy_ms"]
+                res[str(int(qps))] = summarize(lat_dev, lat_host, n, wall, st)
+            log("served %s @%d req/s: %s" % (mode, qps, res[str(int(qps))]))
+        out[mode] = res
+    with Server(budget, copy_to_host=False, auto_release=True) as s:
+        s.run_load(pool[:4], qps=200.0, duration_s=0.2, seed=1)
+        base = s.stats()
+        lat_dev, lat_host, n, wall = s.run_load(pool, weights, closed_loop=closed_loop, duration_s=duration_s,
+                                                seed=7)
+        st = s.stats()
+        for k in ("texels", "algorithmic_bytes"):
+            st[k] -= base[k]
+        st["device_busy_ms"] -= base["device_busy_ms"]
+        out["saturated_closed_loop_%d" % closed_loop] = summarize(lat_dev, lat_host, n, wall, st)
+        log("served saturated: %s" % out["saturated_closed_loop_%d" % closed_loop])
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -200,6 +267,8 @@ def main():
     ap.add_argument("--images", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the per-image comparison")
+    ap.add_argument("--no-served", action="store_true", help="skip the served (Poisson arrivals) leg")
+    ap.add_argument("--served-seconds", type=float, default=3.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device-to-device copy of 512 MiB (5x), the reference kernel for "
                          "scaling FETCH_SIZE / WRITE_SIZE counters to bytes")
@@ -294,6 +363,9 @@ def main():
             singles[name] = {"ms": round(ms_s, 4), "Mtexel_s": round(w * h / (ms_s * 1e-3) / 1e6, 1)}
         result["single_image"] = singles
         log("single images: %s" % singles)
+
+    if not args.no_served:
+        result["served"] = run_served([200.0, 1000.0, 5000.0], args.served_seconds)
 
     if not args.no_cpu:
         rate, threads, passes, secs = run_cpu_baseline(files)

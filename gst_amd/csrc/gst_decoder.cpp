@@ -757,6 +757,14 @@ int gst_batch_error_bits(gst_batch *b, gst_stream stream, uint32_t *bits) {
   return GST_OK;
 }
 
+int gst_batch_error_bits_async(gst_batch *b, gst_stream stream, uint32_t *host_bits) {
+  if (!b || !host_bits) return GST_ERR_INVALID_ARGUMENT;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  GST_HIP(hipMemcpyAsync(host_bits, b->dmem + b->err_off, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  GST_HIP(hipMemsetAsync(b->dmem + b->err_off, 0, sizeof(uint32_t), s));
+  return GST_OK;
+}
+
 size_t gst_batch_device_bytes(const gst_batch *b) { return b ? b->dbytes : 0; }
 
 int gst_batch_destroy(gst_batch *b) {

@@ -192,6 +192,9 @@ int gst_batch_launch_graph(gst_batch *b, gst_stream stream);
  * build_table, ans_decode, decode_indices, inv_wavelet, assemble. Blocking. */
 int gst_batch_decode_timed(gst_batch *b, gst_stream stream, float ms[5]);
 int gst_batch_error_bits(gst_batch *b, gst_stream stream, uint32_t *bits);
+/* Non-blocking form: enqueues on `stream` a copy of the error bits into `host_bits`
+ * (pinned host memory) followed by their reset; read it once `stream` has passed. */
+int gst_batch_error_bits_async(gst_batch *b, gst_stream stream, uint32_t *host_bits);
 /* Device bytes the plan holds (descriptors + tables + scratch). */
 size_t gst_batch_device_bytes(const gst_batch *b);
 int gst_batch_destroy(gst_batch *b);
