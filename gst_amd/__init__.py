@@ -76,6 +76,7 @@ class BatchConfig(ctypes.Structure):
         ("grid_indices", ctypes.c_uint32),
         ("grid_assemble", ctypes.c_uint32),
         ("rgb_output", ctypes.c_uint32),
+        ("wavelet_waves_per_group", ctypes.c_uint32),
     ]
 
 
@@ -275,11 +276,12 @@ class Batch:
     """A device-resident plan for up to `capacity_images` textures of any legal size."""
 
     def __init__(self, capacity_images, capacity_blocks, capacity_palette=0, groups_per_wave=0,
-                 waves_per_group=0, grids=None, rgb=False):
+                 waves_per_group=0, grids=None, rgb=False, wavelet_waves=0):
         grids = grids or {}
         cfg = BatchConfig(capacity_images, capacity_blocks, capacity_palette, groups_per_wave,
                           waves_per_group, grids.get("ans_decode", 0), grids.get("inv_wavelet", 0),
-                          grids.get("decode_indices", 0), grids.get("assemble", 0), 1 if rgb else 0)
+                          grids.get("decode_indices", 0), grids.get("assemble", 0), 1 if rgb else 0,
+                          wavelet_waves)
         self._h = _P()
         _check(lib().gst_batch_create(ctypes.byref(cfg), ctypes.byref(self._h)), "gst_batch_create")
 

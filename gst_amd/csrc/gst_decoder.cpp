@@ -139,6 +139,7 @@ StageConfig DefaultStageConfig(bool rgb) {
   StageConfig s;
   s.ans_waves = 4;
   s.ans_groups_per_wave = 2;
+  s.wavelet_waves = 1;
   s.rgb = rgb ? 1 : 0;
   return s;
 }
@@ -357,7 +358,7 @@ int DecodeStaging(const gst_header *hdrs, uint32_t n, hipStream_t s, const void 
     a.tables = tables;
     a.table_status = status;
     for (uint32_t st = 0; st < gst::kNumStages; ++st)
-      grids[st] = gst::StageUnitsOf(descs[0], st, GroupsPerUnit(cfg));
+      grids[st] = gst::SinglePassGrid(st, gst::StageUnitsOf(descs[0], st, GroupsPerUnit(cfg)), cfg);
     rc = LaunchAll(a, grids, cfg, s);
   } else {
     std::vector<uint8_t> blob;
@@ -367,7 +368,8 @@ int DecodeStaging(const gst_header *hdrs, uint32_t n, hipStream_t s, const void 
     rc = PinnedRing::Get().Upload(d_blob, blob.data(), blob.size(), s);
     if (rc == GST_OK) {
       const KernelArgs a = PlanArgs(d_blob, n, err, tables, status);
-      rc = LaunchAll(a, units, cfg, s);
+      for (uint32_t st = 0; st < gst::kNumStages; ++st) grids[st] = gst::SinglePassGrid(st, units[st], cfg);
+      rc = LaunchAll(a, grids, cfg, s);
     }
   }
   if (sc.async) {
@@ -628,6 +630,7 @@ int gst_batch_create(const gst_batch_config *cfg, gst_batch **out) {
   b->cfg = *cfg;
   b->stage.ans_groups_per_wave = cfg->ans_groups_per_wave ? cfg->ans_groups_per_wave : 2;
   b->stage.ans_waves = cfg->ans_waves_per_group ? cfg->ans_waves_per_group : 4;
+  b->stage.wavelet_waves = cfg->wavelet_waves_per_group ? cfg->wavelet_waves_per_group : 1;
   b->stage.rgb = cfg->rgb_output ? 1 : 0;
   const uint32_t cus = static_cast<uint32_t>(G().cu_count > 0 ? G().cu_count : 256);
   b->grids[gst::kStageTables] = std::min<uint32_t>(4 * cfg->capacity_images, cus);
@@ -815,7 +818,7 @@ int StageLaunch(uint32_t stage, const gst_header *hdr, const uint8_t *d_freqs, c
   a.tables = const_cast<uint32_t *>(d_tables);
   a.table_status = const_cast<uint32_t *>(d_tables) + 4 * gst::kTableSize;
   const StageConfig cfg = DefaultStageConfig(rgb);
-  const uint32_t grid = gst::StageUnitsOf(d, stage, GroupsPerUnit(cfg));
+  const uint32_t grid = gst::SinglePassGrid(stage, gst::StageUnitsOf(d, stage, GroupsPerUnit(cfg)), cfg);
   GST_HIP(gst::LaunchStage(stage, a, grid, cfg, s));
   return GST_OK;
 }

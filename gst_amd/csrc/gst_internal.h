@@ -17,7 +17,7 @@ constexpr uint32_t kGroupSyms = kLanesPerGroup * kSymsPerLane;
 constexpr uint32_t kTileDim = 32;            // codec/codec_base.h:22
 constexpr uint32_t kTileSyms = kTileDim * kTileDim;
 constexpr uint32_t kAssembleBlocksPerUnit = 1024;
-constexpr uint32_t kWaveletTilesPerUnit = 2;  // one wave64, a tile per half-wave
+constexpr uint32_t kWaveletTilesPerUnit = 2;  // a tile per half-wave; a wave takes two units
 
 enum Stage : uint32_t {
   kStageTables = 0,
@@ -99,8 +99,18 @@ struct KernelArgs {
 struct StageConfig {
   uint32_t ans_waves;            // waves per rANS work-group (1, 2, 4, 8)
   uint32_t ans_groups_per_wave;  // 1 or 2
+  uint32_t wavelet_waves;        // waves per wavelet work-group (1, 2, 4); a wave takes 2 units
   uint32_t rgb;                  // 1: RGB8 assembly
 };
+
+// Work-groups that give every unit of `units` its own pass (no grid-stride repeat).
+inline uint32_t SinglePassGrid(uint32_t stage, uint32_t units, const StageConfig &c) {
+  if (stage == kStageWavelet) {
+    const uint32_t per = 2 * c.wavelet_waves;
+    return (units + per - 1) / per;
+  }
+  return units;
+}
 
 // Launches one stage with `grid` work-groups (each loops over units grid-stride).
 hipError_t LaunchStage(uint32_t stage, const KernelArgs &args, uint32_t grid,

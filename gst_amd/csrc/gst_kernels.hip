@@ -391,13 +391,45 @@ __global__ __launch_bounds__(256) void DecodeIndicesKernel(KernelArgs a) {
 // ---------------------------------------------------------------------------
 // Stage 4: inverse 5/3 wavelet (codec/inverse_wavelet.cl:98-276).
 //
-// One wave64 per two 32x32 tiles, one tile per half-wave.  A lane holds a whole
-// row of its tile in 32 registers, so the row pass of each level is straight-line
-// code with compile-time mirror indices; the column pass runs after a transpose
-// through LDS (row stride 33 words: conflict-free in both directions).  Levels
-// len = 2..32 each transform rows y < len over x < len, then columns x < len over
-// y < len, exactly the reference's transposed ping-pong.  The last level leaves
-// lane r holding column r, which is stored as int8 with 32-byte row segments.
+// A wave64 transforms four 32x32 tiles (two work units) at once: lane r of half-wave
+// h holds row r of tile h of the first unit in the low 16 bits and row r of tile h
+// of the second unit in the high 16 bits of 32 registers, so every add and shift is
+// a packed 16-bit op over two tiles.  16 bits reproduce the reference's int32
+// arithmetic exactly: inputs are symbol - 128 in [-128, 127]; a 1-D pass maps
+// low-band bound Lo and high-band bound Hi to at most Lo + 1.5 Hi + 0.5 (the 5/3
+// synthesis taps sum to 1 on the low band and 1.5 on the high band, truncation adds
+// under 1), so a level (rows, then columns) adds at most 674 to the bound, every
+// value after 5 levels stays within 128 + 5 * 674 = 3498 and every sum formed on the
+// way within 2 * 3498 + 2.
+//
+// The row pass of each level is straight-line code with compile-time mirror
+// indices; the column pass runs after a transpose through LDS (row stride 33
+// words: conflict-free in both directions).  Levels len = 2..32 each transform rows
+// y < len over x < len, then columns x < len over y < len, exactly the reference's
+// transposed ping-pong.  A last transpose returns rows, whose low bytes leave as
+// int8 in two 16-byte stores per tile row.
+typedef int16_t s16x2 __attribute__((ext_vector_type(2)));
+typedef uint16_t u16x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ s16x2 AsPair(uint32_t w) { return __builtin_bit_cast(s16x2, w); }
+__device__ __forceinline__ uint32_t Bits(s16x2 v) { return __builtin_bit_cast(uint32_t, v); }
+
+// C's truncating division of each half: negative values take divisor - 1 first.
+__device__ __forceinline__ s16x2 TruncDiv4(s16x2 s) {
+  const s16x2 neg = s >> static_cast<s16x2>(15);  // 0 or -1
+  return (s + neg * static_cast<s16x2>(-3)) >> static_cast<s16x2>(2);
+}
+__device__ __forceinline__ s16x2 TruncDiv2(s16x2 s) {
+  const u16x2 sign = __builtin_bit_cast(u16x2, s) >> static_cast<u16x2>(15);  // 0 or 1
+  return (s + __builtin_bit_cast(s16x2, sign)) >> static_cast<s16x2>(1);
+}
+
+__device__ __forceinline__ void WaveSync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __host__ __device__ constexpr int Mirror(int idx, int range) {
   return (idx - static_cast<int>(idx >= range) * (idx - range + 2)) < 0
              ? -(idx - static_cast<int>(idx >= range) * (idx - range + 2))
@@ -405,20 +437,20 @@ __host__ __device__ constexpr int Mirror(int idx, int range) {
 }
 
 template <int LEN>
-__device__ __forceinline__ void Inverse1D(int (&v)[kTileDim]) {
+__device__ __forceinline__ void Inverse1D(s16x2 (&v)[kTileDim]) {
   constexpr int MID = LEN / 2;
-  int e[LEN];
+  s16x2 e[LEN];
 #pragma unroll
   for (int k = 0; k < MID; ++k) {
     const int p = MID + Mirror(2 * k - 1, LEN) / 2;
     const int n = MID + Mirror(2 * k + 1, LEN) / 2;
-    e[2 * k] = v[k] - (v[p] + v[n] + 2) / 4;
+    e[2 * k] = v[k] - TruncDiv4(v[p] + v[n] + static_cast<s16x2>(2));
   }
 #pragma unroll
   for (int k = 0; k < MID; ++k) {
     const int p = Mirror(2 * k, LEN);
     const int n = Mirror(2 * k + 2, LEN);
-    e[2 * k + 1] = v[MID + k] + (e[p] + e[n]) / 2;
+    e[2 * k + 1] = v[MID + k] + TruncDiv2(e[p] + e[n]);
   }
 #pragma unroll
   for (int i = 0; i < LEN; ++i) v[i] = e[i];
@@ -426,65 +458,111 @@ __device__ __forceinline__ void Inverse1D(int (&v)[kTileDim]) {
 
 constexpr int kLdsStride = kTileDim + 1;
 
+// One level on the half-wave's tile pair; lanes r >= LEN sit out (their rows and
+// columns are outside this level's square).  Each half-wave owns its LDS tile, so
+// wave-level ordering is all the transposes need.
 template <int LEN>
-__device__ __forceinline__ void WaveletLevel(int (&v)[kTileDim], int *L, int r) {
+__device__ __forceinline__ void WaveletLevel(s16x2 (&v)[kTileDim], uint32_t *L, int r) {
   if (r < LEN) {
     Inverse1D<LEN>(v);
 #pragma unroll
-    for (int c = 0; c < LEN; ++c) L[r * kLdsStride + c] = v[c];
+    for (int c = 0; c < LEN; ++c) L[r * kLdsStride + c] = Bits(v[c]);
   }
-  __syncthreads();
+  WaveSync();
   if (r < LEN) {
 #pragma unroll
-    for (int c = 0; c < LEN; ++c) v[c] = L[c * kLdsStride + r];
+    for (int c = 0; c < LEN; ++c) v[c] = AsPair(L[c * kLdsStride + r]);
   }
-  __syncthreads();
-  if (r < LEN) Inverse1D<LEN>(v);
-  if (LEN < static_cast<int>(kTileDim)) {
-    if (r < LEN) {
+  WaveSync();
+  if (r < LEN) {
+    Inverse1D<LEN>(v);
 #pragma unroll
-      for (int c = 0; c < LEN; ++c) L[c * kLdsStride + r] = v[c];
-    }
-    __syncthreads();
-    if (r < LEN) {
-#pragma unroll
-      for (int c = 0; c < LEN; ++c) v[c] = L[r * kLdsStride + c];
-    }
-    __syncthreads();
+    for (int c = 0; c < LEN; ++c) L[c * kLdsStride + r] = Bits(v[c]);
   }
+  WaveSync();
+  if (r < LEN) {
+#pragma unroll
+    for (int c = 0; c < LEN; ++c) v[c] = AsPair(L[r * kLdsStride + c]);
+  }
+  WaveSync();
 }
 
-__global__ __launch_bounds__(64) void InvWaveletKernel(KernelArgs a) {
-  __shared__ int lds[2][kTileDim * kLdsStride];
-  const uint32_t lane = threadIdx.x, half = lane >> 5, r = lane & 31;
+struct TileRef {
+  const uint8_t *src;  // tile-linear symbols (1024 bytes)
+  int8_t *dst;         // top-left of the tile in the raster plane
+  uint32_t pitch;      // plane row length (blocks)
+};
+
+__device__ __forceinline__ TileRef TileAt(const ImageDesc &d, uint32_t tile) {
+  const uint32_t per_plane = d.nv / kTileSyms, tiles_x = d.bx / kTileDim;
+  const uint32_t plane = tile / per_plane, t = tile - plane * per_plane;
+  const uint32_t ty = t / tiles_x, tx = t - ty * tiles_x;
+  const size_t base = static_cast<size_t>(plane) * d.nv;
+  return {d.symbols + base + static_cast<size_t>(t) * kTileSyms,
+          d.planes + base + static_cast<size_t>(ty) * kTileDim * d.bx + tx * kTileDim, d.bx};
+}
+
+// Tile `half` of wavelet unit `u` (uniform address math, one select per lane).
+__device__ __forceinline__ TileRef WaveletTile(const KernelArgs &a, uint32_t u, uint32_t half) {
+  const Located loc = Locate(a, kStageWavelet, u);
+  const ImageDesc d = LoadDesc(a, loc.img);
+  const TileRef t0 = TileAt(d, loc.local * kWaveletTilesPerUnit);
+  const TileRef t1 = TileAt(d, loc.local * kWaveletTilesPerUnit + 1);
+  return half ? t1 : t0;
+}
+
+template <uint32_t WAVES>
+__global__ __launch_bounds__(WAVES * 64) void InvWaveletKernel(KernelArgs a) {
+  __shared__ uint32_t lds[WAVES][2][kTileDim * kLdsStride];
+  const uint32_t lane = threadIdx.x & 63, half = lane >> 5, r = lane & 31;
+  const uint32_t wave = Uniform(threadIdx.x >> 6);
   const uint32_t total = StageTotal(a, kStageWavelet, 1);
-  for (uint32_t u = blockIdx.x; u < total; u += gridDim.x) {
-    const Located loc = Locate(a, kStageWavelet, u);
-    const ImageDesc d = LoadDesc(a, loc.img);
-    const uint32_t tiles_per_plane = d.nv / kTileSyms;
-    const uint32_t tile = loc.local * kWaveletTilesPerUnit + half;
-    const uint32_t plane = tile / tiles_per_plane, t = tile % tiles_per_plane;
-    const uint8_t *src = d.symbols + static_cast<size_t>(plane) * d.nv + static_cast<size_t>(t) * kTileSyms + r * kTileDim;
-    const uint4 q0 = reinterpret_cast<const uint4 *>(src)[0];
-    const uint4 q1 = reinterpret_cast<const uint4 *>(src)[1];
-    const uint32_t words[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
-    int v[kTileDim];
+  uint32_t *L = lds[wave][half];
+  for (uint32_t pair = blockIdx.x * WAVES + wave; 2 * pair < total; pair += gridDim.x * WAVES) {
+    const TileRef lo = WaveletTile(a, 2 * pair, half);
+    const bool has_hi = 2 * pair + 1 < total;
+    const TileRef hi = has_hi ? WaveletTile(a, 2 * pair + 1, half) : lo;
+    const uint4 *sa = reinterpret_cast<const uint4 *>(lo.src + r * kTileDim);
+    const uint4 a0 = sa[0], a1 = sa[1];
+    uint4 b0 = make_uint4(0, 0, 0, 0), b1 = b0;
+    if (has_hi) {
+      const uint4 *sb = reinterpret_cast<const uint4 *>(hi.src + r * kTileDim);
+      b0 = sb[0];
+      b1 = sb[1];
+    }
+    const uint32_t wa[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+    const uint32_t wb[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
+    s16x2 v[kTileDim];
 #pragma unroll
-    for (int i = 0; i < static_cast<int>(kTileDim); ++i)
-      v[i] = static_cast<int>((words[i >> 2] >> (8 * (i & 3))) & 0xFFu) - 128;
-    int *L = lds[half];
+    for (int i = 0; i < static_cast<int>(kTileDim); ++i) {
+      // Byte i&3 of the first tile's word to bits 0-7, of the second's to bits
+      // 16-23 (selector 0x0c is a zero byte), then remove the +128 bias.
+      const uint32_t q = static_cast<uint32_t>(i & 3);
+      const uint32_t sel = 0x0C000C00u | q | ((4u + q) << 16);
+      v[i] = AsPair(__builtin_amdgcn_perm(wb[i >> 2], wa[i >> 2], sel)) - static_cast<s16x2>(128);
+    }
     WaveletLevel<2>(v, L, static_cast<int>(r));
     WaveletLevel<4>(v, L, static_cast<int>(r));
     WaveletLevel<8>(v, L, static_cast<int>(r));
     WaveletLevel<16>(v, L, static_cast<int>(r));
     WaveletLevel<32>(v, L, static_cast<int>(r));
-    const uint32_t tiles_x = d.bx / kTileDim;
-    const uint32_t tx = t % tiles_x, ty = t / tiles_x;
-    int8_t *dst = d.planes + static_cast<size_t>(plane) * d.nv +
-                  static_cast<size_t>(ty * kTileDim) * d.bx + tx * kTileDim + r;
+    uint32_t oa[8], ob[8];
 #pragma unroll
-    for (int y = 0; y < static_cast<int>(kTileDim); ++y)
-      dst[static_cast<size_t>(y) * d.bx] = static_cast<int8_t>(v[y]);
+    for (int w = 0; w < 8; ++w) {
+      // Low bytes of four pairs: t01 = (A0, A1, B0, B1), t23 = (A2, A3, B2, B3).
+      const uint32_t t01 = __builtin_amdgcn_perm(Bits(v[4 * w + 1]), Bits(v[4 * w]), 0x06020400u);
+      const uint32_t t23 = __builtin_amdgcn_perm(Bits(v[4 * w + 3]), Bits(v[4 * w + 2]), 0x06020400u);
+      oa[w] = __builtin_amdgcn_perm(t23, t01, 0x05040100u);
+      ob[w] = __builtin_amdgcn_perm(t23, t01, 0x07060302u);
+    }
+    uint4 *da = reinterpret_cast<uint4 *>(lo.dst + static_cast<size_t>(r) * lo.pitch);
+    da[0] = make_uint4(oa[0], oa[1], oa[2], oa[3]);
+    da[1] = make_uint4(oa[4], oa[5], oa[6], oa[7]);
+    if (has_hi) {
+      uint4 *db = reinterpret_cast<uint4 *>(hi.dst + static_cast<size_t>(r) * hi.pitch);
+      db[0] = make_uint4(ob[0], ob[1], ob[2], ob[3]);
+      db[1] = make_uint4(ob[4], ob[5], ob[6], ob[7]);
+    }
   }
 }
 
@@ -635,7 +713,14 @@ hipError_t LaunchStage(uint32_t stage, const KernelArgs &args, uint32_t grid,
       hipLaunchKernelGGL(DecodeIndicesKernel, dim3(grid), dim3(256), 0, stream, args);
       return hipGetLastError();
     case kStageWavelet:
-      hipLaunchKernelGGL(InvWaveletKernel, dim3(grid), dim3(64), 0, stream, args);
+      if (cfg.wavelet_waves == 1)
+        hipLaunchKernelGGL(InvWaveletKernel<1>, dim3(grid), dim3(64), 0, stream, args);
+      else if (cfg.wavelet_waves == 2)
+        hipLaunchKernelGGL(InvWaveletKernel<2>, dim3(grid), dim3(128), 0, stream, args);
+      else if (cfg.wavelet_waves == 4)
+        hipLaunchKernelGGL(InvWaveletKernel<4>, dim3(grid), dim3(256), 0, stream, args);
+      else
+        return hipErrorInvalidValue;
       return hipGetLastError();
     case kStageAssemble:
       if (cfg.rgb)

@@ -175,6 +175,8 @@ typedef struct gst_batch_config {
   uint32_t grid_indices;
   uint32_t grid_assemble;
   uint32_t rgb_output;        /* 1: RGB8 assembly instead of DXT1 */
+  uint32_t wavelet_waves_per_group; /* waves per work-group of the wavelet kernel, each
+                                       transforming 4 tiles at a time: 1, 2 or 4; 0 = 1 */
 } gst_batch_config;
 
 int gst_batch_create(const gst_batch_config *cfg, gst_batch **out);
