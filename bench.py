@@ -109,16 +109,22 @@ def check_outputs(gpu, d_out, offsets, dxts, what):
             raise SystemExit("%s: image %d differs from the expected DXT1 blocks" % (what, i))
 
 
-def run_batched(gpu, files, dxts, steps, warmup, gpw=0, waves=0, grids=None):
+def run_batched(gpu, files, dxts, steps, warmup, tune=True):
     import gst_amd
     hdrs = [gst_amd.ParseHeader(f) for f in files]
     d_files, file_ptrs = gpu.upload_many(files)
     out_sizes = [h.dxt_bytes for h in hdrs]
     out_offs = list(np.cumsum([0] + out_sizes[:-1]))
     d_out = gpu.empty(sum(out_sizes))
-    b = gst_amd.Batch(len(hdrs), sum(h.num_blocks for h in hdrs), sum(h.palette_bytes for h in hdrs),
-                      groups_per_wave=gpw, waves_per_group=waves, grids=grids)
+    b = gst_amd.Batch(len(hdrs), sum(h.num_blocks for h in hdrs), sum(h.palette_bytes for h in hdrs))
     b.bind(hdrs, file_ptrs, [d_out.data_ptr() + int(o) for o in out_offs], gpu.stream)
+    tuned = None
+    if tune:
+        t0 = time.perf_counter()
+        tuned = b.autotune(gpu.stream, reps=5)
+        tuned["seconds"] = round(time.perf_counter() - t0, 2)
+        log("autotune: %s" % tuned)
+        d_out.zero_()
     b.decode(gpu.stream)
     check_outputs(gpu, d_out, out_offs, dxts, "batched eager")
     b.capture_graph(gpu.stream)
@@ -139,7 +145,7 @@ def run_batched(gpu, files, dxts, steps, warmup, gpw=0, waves=0, grids=None):
     dev_bytes = b.device_bytes
     b.close()
     del d_files
-    return hdrs, ms, stage, dev_bytes
+    return hdrs, ms, stage, dev_bytes, tuned
 
 
 def run_plain(gpu, files, dxts, passes, warmup):
@@ -196,9 +202,13 @@ SERVED_MIX = [((512, 512), 0.50, 16), ((1024, 1024), 0.30, 12), ((2048, 2048), 0
               ((4096, 4096), 0.05, 2)]
 
 
-def run_served(qps_list, duration_s, budget_gib=32, closed_loop=64):
+def run_served(qps_list, duration_s, budget_gib=32, closed_loop=64, tuned=None):
     """Poisson arrivals of the size mix through the continuous-batching server and
-    through the plain per-request server; latency is arrival -> output in HBM."""
+    through the plain per-request server; latency is arrival -> output in HBM.  The
+    batched server takes the rANS launch shape the batched leg's autotune chose."""
+    shape = {}
+    if tuned:
+        shape = {"groups_per_wave": tuned["ans_groups_per_wave"], "waves_per_group": tuned["ans_waves_per_group"]}
     from gst_amd import encode
     from gst_amd.serve import Server
     specs, weights = [], []
@@ -233,7 +243,8 @@ def run_served(qps_list, duration_s, budget_gib=32, closed_loop=64):
     for mode in ("batched", "plain"):
         res = {}
         for qps in qps_list:
-            with Server(budget, plain=(mode == "plain"), copy_to_host=True, auto_release=True) as s:
+            with Server(budget, plain=(mode == "plain"), copy_to_host=True, auto_release=True,
+                        **(shape if mode == "batched" else {})) as s:
                 s.run_load(pool[:4], qps=200.0, duration_s=0.2, seed=1)  # warm-up (graphs, pools)
                 base = s.stats()
                 lat_dev, lat_host, n, wall = s.run_load(pool, weights, qps=qps, duration_s=duration_s,
@@ -245,7 +256,7 @@ def run_served(qps_list, duration_s, budget_gib=32, closed_loop=64):
                 res[str(int(qps))] = summarize(lat_dev, lat_host, n, wall, st)
             log("served %s @%d req/s: %s" % (mode, qps, res[str(int(qps))]))
         out[mode] = res
-    with Server(budget, copy_to_host=False, auto_release=True) as s:
+    with Server(budget, copy_to_host=False, auto_release=True, **shape) as s:
         s.run_load(pool[:4], qps=200.0, duration_s=0.2, seed=1)
         base = s.stats()
         lat_dev, lat_host, n, wall = s.run_load(pool, weights, closed_loop=closed_loop, duration_s=duration_s,
@@ -269,6 +280,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the per-image comparison")
     ap.add_argument("--no-served", action="store_true", help="skip the served (Poisson arrivals) leg")
     ap.add_argument("--served-seconds", type=float, default=3.0)
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes (no autotune)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device-to-device copy of 512 MiB (5x), the reference kernel for "
                          "scaling FETCH_SIZE / WRITE_SIZE counters to bytes")
@@ -298,7 +310,8 @@ def main():
     files, dxts = encode.make_textures(specs, nthreads=min(16, os.cpu_count() or 1))
     log("corpus: %d textures encoded in %.1fs" % (len(files), time.perf_counter() - t0))
 
-    hdrs, ms, stage, dev_bytes = run_batched(gpu, files, dxts, args.steps, args.warmup)
+    hdrs, ms, stage, dev_bytes, tuned = run_batched(gpu, files, dxts, args.steps, args.warmup,
+                                                    tune=not args.no_tune)
     texels = sum(h.width * h.height for h in hdrs)
     value = texels / (ms * 1e-3) / 1e6
     log("batched: %.3f ms/step  %.0f Mtexel/s  stages %s" % (ms, value, stage))
@@ -345,6 +358,7 @@ def main():
             "B_min_bytes": b_min,
             "B_min_frac": round(b_min / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4),
         },
+        "autotune": tuned,
         "stage_ms": {k: round(v, 4) for k, v in stage.items()},
         "stage_gbs": {k: round(sb[k] / (stage[k] * 1e-3) / 1e9, 1) for k in stage},
     }
@@ -365,7 +379,7 @@ def main():
         log("single images: %s" % singles)
 
     if not args.no_served:
-        result["served"] = run_served([200.0, 1000.0, 5000.0], args.served_seconds)
+        result["served"] = run_served([200.0, 1000.0, 5000.0], args.served_seconds, tuned=tuned)
 
     if not args.no_cpu:
         rate, threads, passes, secs = run_cpu_baseline(files)

@@ -80,6 +80,20 @@ class BatchConfig(ctypes.Structure):
     ]
 
 
+class TuneResult(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_uint32) for name in (
+        "ans_groups_per_wave", "ans_waves_per_group", "wavelet_waves_per_group", "grid_ans",
+        "grid_indices", "grid_wavelet", "grid_assemble", "candidates")] + [
+        ("stage_ms", ctypes.c_float * 5), ("total_ms", ctypes.c_float)]
+
+    def as_dict(self):
+        d = {name: getattr(self, name) for name, _ in self._fields_[:8]}
+        d["stage_ms"] = dict(zip(STAGES, list(self.stage_ms)))
+        d["total_ms"] = self.total_ms
+        d["streams_per_wavefront"] = 32 * self.ans_groups_per_wave
+        return d
+
+
 _lib = None
 _P = ctypes.c_void_p
 _U32 = ctypes.c_uint32
@@ -116,6 +130,7 @@ def _declare(lib):
         "gst_batch_launch_graph": ([_P, _P], ctypes.c_int),
         "gst_batch_decode_timed": ([_P, _P, ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
         "gst_batch_error_bits": ([_P, _P, ctypes.POINTER(_U32)], ctypes.c_int),
+        "gst_batch_autotune": ([_P, _P, _U32, ctypes.POINTER(TuneResult)], ctypes.c_int),
         "gst_batch_device_bytes": ([_P], _SZ),
         "gst_batch_destroy": ([_P], ctypes.c_int),
     }
@@ -311,6 +326,12 @@ class Batch:
         bits = _U32()
         _check(lib().gst_batch_error_bits(self._h, stream, ctypes.byref(bits)), "gst_batch_error_bits")
         return bits.value
+
+    def autotune(self, stream, reps=5):
+        """Times every launch shape on the bound batch and keeps the fastest per stage."""
+        res = TuneResult()
+        _check(lib().gst_batch_autotune(self._h, stream, reps, ctypes.byref(res)), "gst_batch_autotune")
+        return res.as_dict()
 
     @property
     def device_bytes(self):

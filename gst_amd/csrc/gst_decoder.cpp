@@ -606,6 +606,7 @@ struct gst_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   uint32_t n_bound = 0;
+  std::vector<ImageDesc> bound;  // descriptors of the bound images (for re-planning)
 };
 
 namespace {
@@ -615,6 +616,54 @@ KernelArgs BatchArgs(gst_batch *b) {
                   reinterpret_cast<uint32_t *>(b->dmem + b->err_off),
                   reinterpret_cast<uint32_t *>(b->dmem + b->tables_off),
                   reinterpret_cast<uint32_t *>(b->dmem + b->status_off));
+}
+
+// Builds the plan of the bound images for the batch's rANS work-unit size and
+// copies it to the device on `s` through the pinned staging buffer.
+int UploadPlan(gst_batch *b, hipStream_t s, uint32_t units[gst::kNumStages]) {
+  std::vector<uint8_t> blob;
+  BuildPlan(b->bound, b->cfg.capacity_images, GroupsPerUnit(b->stage), &blob, units);
+  if (b->blob_pending) GST_HIP(hipEventSynchronize(b->blob_ev));
+  memcpy(b->h_blob, blob.data(), blob.size());
+  GST_HIP(hipMemcpyAsync(b->dmem + b->plan_off, b->h_blob, blob.size(), hipMemcpyHostToDevice, s));
+  GST_HIP(hipEventRecord(b->blob_ev, s));
+  b->blob_pending = true;
+  return GST_OK;
+}
+
+void DropGraph(gst_batch *b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+}
+
+// Median time (ms) of `reps` back-to-back launches of one stage, after one
+// untimed launch.
+int TimeStage(gst_batch *b, uint32_t st, uint32_t grid, const StageConfig &cfg, hipStream_t s,
+              uint32_t reps, float *ms) {
+  const KernelArgs a = BatchArgs(b);
+  std::vector<hipEvent_t> ev(reps + 1, nullptr);
+  int rc = GST_OK;
+  hipError_t e = gst::LaunchStage(st, a, grid, cfg, s);
+  for (size_t i = 0; i < ev.size() && e == hipSuccess; ++i) e = hipEventCreate(&ev[i]);
+  if (e == hipSuccess) e = hipEventRecord(ev[0], s);
+  for (uint32_t r = 0; r < reps && e == hipSuccess; ++r) {
+    e = gst::LaunchStage(st, a, grid, cfg, s);
+    if (e == hipSuccess) e = hipEventRecord(ev[r + 1], s);
+  }
+  if (e == hipSuccess) e = hipEventSynchronize(ev[reps]);
+  std::vector<float> t(reps, 0.f);
+  for (uint32_t r = 0; r < reps && e == hipSuccess; ++r) e = hipEventElapsedTime(&t[r], ev[r], ev[r + 1]);
+  if (e != hipSuccess) {
+    rc = HipFail(e);
+  } else {
+    std::sort(t.begin(), t.end());
+    *ms = t[reps / 2];
+  }
+  for (hipEvent_t x : ev)
+    if (x) (void)hipEventDestroy(x);
+  return rc;
 }
 
 }  // namespace
@@ -690,15 +739,94 @@ int gst_batch_bind(gst_batch *b, const gst_header *hdrs, uint32_t n, const void 
              static_cast<uint8_t *>(d_outs[i]), b->stage.rgb != 0, &descs[i]);
     off += c.total;
   }
-  std::vector<uint8_t> blob;
+  b->bound = std::move(descs);
   uint32_t units[gst::kNumStages];
-  BuildPlan(descs, b->cfg.capacity_images, GroupsPerUnit(b->stage), &blob, units);
-  if (b->blob_pending) GST_HIP(hipEventSynchronize(b->blob_ev));
-  memcpy(b->h_blob, blob.data(), blob.size());
-  GST_HIP(hipMemcpyAsync(b->dmem + b->plan_off, b->h_blob, blob.size(), hipMemcpyHostToDevice, s));
-  GST_HIP(hipEventRecord(b->blob_ev, s));
-  b->blob_pending = true;
+  const int rc = UploadPlan(b, s, units);
+  if (rc != GST_OK) return rc;
   b->n_bound = n;
+  return GST_OK;
+}
+
+int gst_batch_autotune(gst_batch *b, gst_stream stream, uint32_t reps, gst_tune_result *out) {
+  if (!b || b->n_bound == 0) return GST_ERR_INVALID_ARGUMENT;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  reps = std::max<uint32_t>(reps ? reps : 5, 1);
+  const uint32_t cus = static_cast<uint32_t>(G().cu_count > 0 ? G().cu_count : 256);
+  gst_tune_result res;
+  memset(&res, 0, sizeof(res));
+  StageConfig best = b->stage;
+  uint32_t grids[gst::kNumStages];
+  memcpy(grids, b->grids, sizeof(grids));
+  uint32_t units[gst::kNumStages];
+  int rc;
+  float ms = 0.f;
+
+  // Tables: every later stage reads them; one grid per table is the natural shape.
+  if ((rc = TimeStage(b, gst::kStageTables, grids[gst::kStageTables], best, s, reps, &ms)) != GST_OK) return rc;
+  res.stage_ms[gst::kStageTables] = ms;
+
+  // rANS: streams per wavefront (32 = one group per wave, 64 = two) x waves per
+  // work-group x persistent grid; each shape needs its own plan (units of W*GPW groups).
+  static const uint32_t kAnsShapes[][2] = {{1, 2}, {1, 4}, {1, 8}, {2, 1}, {2, 2}, {2, 4}, {2, 8}};
+  float best_ms = 1e30f;
+  for (const auto &shape : kAnsShapes) {
+    StageConfig c = best;
+    c.ans_groups_per_wave = shape[0];
+    c.ans_waves = shape[1];
+    b->stage = c;
+    if ((rc = UploadPlan(b, s, units)) != GST_OK) return rc;
+    const uint32_t cand[] = {cus, 2 * cus, 4 * cus, 8 * cus, units[gst::kStageAns]};
+    for (uint32_t g : cand) {
+      if ((rc = TimeStage(b, gst::kStageAns, g, c, s, reps, &ms)) != GST_OK) return rc;
+      ++res.candidates;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best.ans_groups_per_wave = c.ans_groups_per_wave;
+        best.ans_waves = c.ans_waves;
+        grids[gst::kStageAns] = g;
+      }
+    }
+  }
+  b->stage = best;
+  if ((rc = UploadPlan(b, s, units)) != GST_OK) return rc;
+  // Leave valid symbols behind for the stages that read them.
+  if ((rc = TimeStage(b, gst::kStageAns, grids[gst::kStageAns], best, s, reps, &ms)) != GST_OK) return rc;
+  res.stage_ms[gst::kStageAns] = ms;
+
+  // Index scan and assembly: persistent grid only.  Wavelet: waves per work-group
+  // (4 tiles per wave) x grid.
+  for (uint32_t st : {gst::kStageIndices, gst::kStageWavelet, gst::kStageAssemble}) {
+    best_ms = 1e30f;
+    const uint32_t wave_opts = st == gst::kStageWavelet ? 3 : 1;
+    for (uint32_t wi = 0; wi < wave_opts; ++wi) {
+      StageConfig c = best;
+      if (st == gst::kStageWavelet) c.wavelet_waves = 1u << wi;
+      const uint32_t cand[] = {2 * cus, 4 * cus, 8 * cus, 16 * cus, gst::SinglePassGrid(st, units[st], c)};
+      for (uint32_t g : cand) {
+        if ((rc = TimeStage(b, st, g, c, s, reps, &ms)) != GST_OK) return rc;
+        ++res.candidates;
+        if (ms < best_ms) {
+          best_ms = ms;
+          best.wavelet_waves = c.wavelet_waves;
+          grids[st] = g;
+        }
+      }
+    }
+    res.stage_ms[st] = best_ms;
+  }
+
+  b->stage = best;
+  memcpy(b->grids, grids, sizeof(grids));
+  DropGraph(b);  // a captured graph holds the old shapes
+  res.ans_groups_per_wave = best.ans_groups_per_wave;
+  res.ans_waves_per_group = best.ans_waves;
+  res.wavelet_waves_per_group = best.wavelet_waves;
+  res.grid_ans = grids[gst::kStageAns];
+  res.grid_indices = grids[gst::kStageIndices];
+  res.grid_wavelet = grids[gst::kStageWavelet];
+  res.grid_assemble = grids[gst::kStageAssemble];
+  for (float t : res.stage_ms) res.total_ms += t;
+  if (out) *out = res;
   return GST_OK;
 }
 

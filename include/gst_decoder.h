@@ -197,6 +197,23 @@ int gst_batch_error_bits(gst_batch *b, gst_stream stream, uint32_t *bits);
 /* Non-blocking form: enqueues on `stream` a copy of the error bits into `host_bits`
  * (pinned host memory) followed by their reset; read it once `stream` has passed. */
 int gst_batch_error_bits_async(gst_batch *b, gst_stream stream, uint32_t *host_bits);
+/* Launch shapes chosen by gst_batch_autotune and the stage times they gave. */
+typedef struct gst_tune_result {
+  uint32_t ans_groups_per_wave;      /* 1: 32 streams per wavefront, 2: 64 */
+  uint32_t ans_waves_per_group;
+  uint32_t wavelet_waves_per_group;  /* 4 tiles per wave */
+  uint32_t grid_ans, grid_indices, grid_wavelet, grid_assemble;
+  uint32_t candidates;               /* shapes timed */
+  float stage_ms[5];                 /* median time of the chosen shape per stage */
+  float total_ms;
+} gst_tune_result;
+/* No reference counterpart.  Times every launch shape of every stage on the bound
+ * batch (median of `reps` launches each, 0 = 5) and keeps the fastest per stage:
+ * rANS streams per wavefront {32, 64} x waves per work-group {1, 2, 4, 8} x grid,
+ * wavelet waves per work-group {1, 2, 4} x grid, and the grids of the index scan and
+ * the assembly.  The bound images are decoded along the way.  Drops a captured graph
+ * (capture again afterwards).  Blocking. */
+int gst_batch_autotune(gst_batch *b, gst_stream stream, uint32_t reps, gst_tune_result *out);
 /* Device bytes the plan holds (descriptors + tables + scratch). */
 size_t gst_batch_device_bytes(const gst_batch *b);
 int gst_batch_destroy(gst_batch *b);

@@ -46,12 +46,13 @@ def test_uneven_batch_matches_oracle(device):
         b.close()
 
 
-@pytest.mark.parametrize("gpw,waves", [(2, 4), (1, 4), (2, 8), (2, 1)])
-def test_rans_launch_shapes_agree(device, gpw, waves):
+@pytest.mark.parametrize("gpw,waves,wwaves", [(2, 4, 1), (1, 4, 2), (2, 8, 4), (2, 1, 1), (1, 2, 4)])
+def test_launch_shapes_agree(device, gpw, waves, wwaves):
     files, dxts = encode.make_textures(MIXED[:4])
     blocks, pal = _capacity(files)
     b = gst_amd.Batch(len(files), blocks, pal, groups_per_wave=gpw, waves_per_group=waves,
-                      grids={"ans_decode": 64, "inv_wavelet": 128, "decode_indices": 16, "assemble": 32})
+                      grids={"ans_decode": 64, "inv_wavelet": 128, "decode_indices": 16, "assemble": 32},
+                      wavelet_waves=wwaves)
     try:
         _, _, d_outs = _bind(device, b, files)
         b.decode(device.stream)
@@ -118,5 +119,30 @@ def test_bind_rejects_batches_over_capacity(device):
         with pytest.raises(gst_amd.GstError) as e:
             _bind(device, b, files)
         assert e.value.code == -4
+    finally:
+        b.close()
+
+
+def test_autotune_keeps_results_bit_exact(device):
+    files, dxts = encode.make_textures(MIXED)
+    blocks, pal = _capacity(files)
+    b = gst_amd.Batch(len(files), blocks, pal)
+    try:
+        _, _, outs = _bind(device, b, files)
+        tuned = b.autotune(device.stream, reps=3)
+        assert tuned["streams_per_wavefront"] in (32, 64)
+        assert tuned["wavelet_waves_per_group"] in (1, 2, 4)
+        assert tuned["candidates"] >= 7 * 5 + 3 * 5 + 2 * 5
+        assert all(v > 0 for v in tuned["stage_ms"].values())
+        for o in outs:
+            o.zero_()
+        b.decode(device.stream)
+        assert [device.download(o) for o in outs] == dxts
+        for o in outs:
+            o.zero_()
+        b.capture_graph(device.stream)
+        b.launch_graph(device.stream)
+        assert [device.download(o) for o in outs] == dxts
+        assert b.error_bits(device.stream) == 0
     finally:
         b.close()
