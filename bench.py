@@ -41,18 +41,20 @@ def corpus_c3(n=128, size_seed=42):
     return [(1000 + i, SIZES_C3[p][0], SIZES_C3[p][1]) for i, p in enumerate(picks)]
 
 
-def stage_bytes(hdrs):
-    """Algorithmic read+write bytes per stage for one decode of `hdrs`."""
+def stage_bytes(hdrs, fused=False):
+    """Algorithmic read+write bytes per stage for one decode of `hdrs`.  With the index
+    scan fused into the assembly, the assembly reads the nv diff bytes (plus nv/64
+    bytes of lane totals) instead of nv int32 indices, and the scan stage moves nothing."""
     nv = sum(h.num_blocks for h in hdrs)
     P = sum(h.palette_bytes for h in hdrs)
     S = sum(h.stream_bytes for h in hdrs)
     n = len(hdrs)
     return {
         "build_table": 2048 * n + 32768 * n,
-        "ans_decode": (S + 32768 * n) + (7 * nv + P),
-        "decode_indices": nv + 4 * nv,
+        "ans_decode": (S + 32768 * n) + (7 * nv + P) + (nv // 64 if fused else 0),
+        "decode_indices": 0 if fused else nv + 4 * nv,
         "inv_wavelet": 6 * nv + 6 * nv,
-        "assemble": (6 * nv + 4 * nv + P) + 8 * nv,
+        "assemble": (6 * nv + nv + nv // 64 + P) + 8 * nv if fused else (6 * nv + 4 * nv + P) + 8 * nv,
     }
 
 
@@ -316,7 +318,8 @@ def main():
     value = texels / (ms * 1e-3) / 1e6
     log("batched: %.3f ms/step  %.0f Mtexel/s  stages %s" % (ms, value, stage))
 
-    sb = stage_bytes(hdrs)
+    fused = tuned["assembly_waves_per_group"] > 0 if tuned else True  # the batch default is fused
+    sb = stage_bytes(hdrs, fused)
     dom = max(stage, key=stage.get)
     achieved = sb[dom] / (stage[dom] * 1e-3) / 1e9
     b_pipe, b_min = pipeline_bytes(hdrs)
@@ -360,7 +363,8 @@ def main():
         },
         "autotune": tuned,
         "stage_ms": {k: round(v, 4) for k, v in stage.items()},
-        "stage_gbs": {k: round(sb[k] / (stage[k] * 1e-3) / 1e9, 1) for k in stage},
+        "stage_gbs": {k: round(sb[k] / (stage[k] * 1e-3) / 1e9, 1) if sb[k] else None for k in stage},
+        "index_scan": "fused into the assembly" if fused else "separate launch",
     }
 
     if not args.no_plain:

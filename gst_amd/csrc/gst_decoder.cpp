@@ -132,6 +132,9 @@ void FillDesc(const gst_header &h, const uint8_t *freqs, const uint8_t *streams,
   d->planes = reinterpret_cast<int8_t *>(scratch + c.planes);
   d->indices = reinterpret_cast<int32_t *>(scratch + c.indices);
   d->group_sums = reinterpret_cast<int32_t *>(scratch + c.group_sums);
+  // The rANS stage leaves per-lane sums at the head of the index region; the
+  // separate index scan overwrites them afterwards, the fused assembly reads them.
+  d->lane_sums = d->indices;
   d->out = out;
 }
 
@@ -140,6 +143,7 @@ StageConfig DefaultStageConfig(bool rgb) {
   s.ans_waves = 4;
   s.ans_groups_per_wave = 2;
   s.wavelet_waves = 1;
+  s.fused_waves = rgb ? 0 : 4;
   s.rgb = rgb ? 1 : 0;
   return s;
 }
@@ -680,9 +684,12 @@ int gst_batch_create(const gst_batch_config *cfg, gst_batch **out) {
   b->stage.ans_groups_per_wave = cfg->ans_groups_per_wave ? cfg->ans_groups_per_wave : 2;
   b->stage.ans_waves = cfg->ans_waves_per_group ? cfg->ans_waves_per_group : 4;
   b->stage.wavelet_waves = cfg->wavelet_waves_per_group ? cfg->wavelet_waves_per_group : 1;
+  b->stage.fused_waves = (cfg->rgb_output || cfg->separate_index_scan)
+                             ? 0
+                             : (cfg->assembly_waves_per_group ? cfg->assembly_waves_per_group : 4);
   b->stage.rgb = cfg->rgb_output ? 1 : 0;
   const uint32_t cus = static_cast<uint32_t>(G().cu_count > 0 ? G().cu_count : 256);
-  b->grids[gst::kStageTables] = std::min<uint32_t>(4 * cfg->capacity_images, cus);
+  b->grids[gst::kStageTables] = 4 * cfg->capacity_images;
   b->grids[gst::kStageAns] = cfg->grid_ans ? cfg->grid_ans : cus * 8;
   b->grids[gst::kStageIndices] = cfg->grid_indices ? cfg->grid_indices : cus * 4;
   b->grids[gst::kStageWavelet] = cfg->grid_wavelet ? cfg->grid_wavelet : cus * 16;
@@ -793,26 +800,77 @@ int gst_batch_autotune(gst_batch *b, gst_stream stream, uint32_t reps, gst_tune_
   if ((rc = TimeStage(b, gst::kStageAns, grids[gst::kStageAns], best, s, reps, &ms)) != GST_OK) return rc;
   res.stage_ms[gst::kStageAns] = ms;
 
-  // Index scan and assembly: persistent grid only.  Wavelet: waves per work-group
-  // (4 tiles per wave) x grid.
-  for (uint32_t st : {gst::kStageIndices, gst::kStageWavelet, gst::kStageAssemble}) {
-    best_ms = 1e30f;
-    const uint32_t wave_opts = st == gst::kStageWavelet ? 3 : 1;
-    for (uint32_t wi = 0; wi < wave_opts; ++wi) {
+  // Wavelet: waves per work-group (4 tiles per wave) x grid.
+  best_ms = 1e30f;
+  for (uint32_t w : {1u, 2u, 4u}) {
+    StageConfig c = best;
+    c.wavelet_waves = w;
+    const uint32_t cand[] = {2 * cus, 4 * cus, 8 * cus, 16 * cus,
+                             gst::SinglePassGrid(gst::kStageWavelet, units[gst::kStageWavelet], c)};
+    for (uint32_t g : cand) {
+      if ((rc = TimeStage(b, gst::kStageWavelet, g, c, s, reps, &ms)) != GST_OK) return rc;
+      ++res.candidates;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best.wavelet_waves = w;
+        grids[gst::kStageWavelet] = g;
+      }
+    }
+  }
+  res.stage_ms[gst::kStageWavelet] = best_ms;
+
+  // Index scan + assembly, fused (waves per work-group x grid) or as two launches
+  // (grid each).  The fused shapes go first: they read the lane totals the rANS stage
+  // left at the head of the index region, which the separate scan overwrites.
+  float fused_ms = 1e30f;
+  uint32_t fused_w = 0, fused_grid = 0;
+  if (!best.rgb) {
+    for (uint32_t w : {1u, 2u, 4u}) {
       StageConfig c = best;
-      if (st == gst::kStageWavelet) c.wavelet_waves = 1u << wi;
-      const uint32_t cand[] = {2 * cus, 4 * cus, 8 * cus, 16 * cus, gst::SinglePassGrid(st, units[st], c)};
+      c.fused_waves = w;
+      const uint32_t cand[] = {2 * cus, 4 * cus, 8 * cus, 16 * cus,
+                               gst::SinglePassGrid(gst::kStageAssemble, units[gst::kStageAssemble], c)};
       for (uint32_t g : cand) {
-        if ((rc = TimeStage(b, st, g, c, s, reps, &ms)) != GST_OK) return rc;
+        if ((rc = TimeStage(b, gst::kStageAssemble, g, c, s, reps, &ms)) != GST_OK) return rc;
         ++res.candidates;
-        if (ms < best_ms) {
-          best_ms = ms;
-          best.wavelet_waves = c.wavelet_waves;
-          grids[st] = g;
+        if (ms < fused_ms) {
+          fused_ms = ms;
+          fused_w = w;
+          fused_grid = g;
         }
       }
     }
-    res.stage_ms[st] = best_ms;
+  }
+  float sep_ms[2] = {1e30f, 1e30f};
+  uint32_t sep_grid[2] = {0, 0};
+  {
+    StageConfig c = best;
+    c.fused_waves = 0;
+    const uint32_t st2[2] = {gst::kStageIndices, gst::kStageAssemble};
+    for (int i = 0; i < 2; ++i) {
+      const uint32_t cand[] = {2 * cus, 4 * cus, 8 * cus, 16 * cus, units[st2[i]]};
+      for (uint32_t g : cand) {
+        if ((rc = TimeStage(b, st2[i], g, c, s, reps, &ms)) != GST_OK) return rc;
+        ++res.candidates;
+        if (ms < sep_ms[i]) {
+          sep_ms[i] = ms;
+          sep_grid[i] = g;
+        }
+      }
+    }
+  }
+  if (fused_ms < sep_ms[0] + sep_ms[1]) {
+    best.fused_waves = fused_w;
+    grids[gst::kStageIndices] = 0;
+    grids[gst::kStageAssemble] = fused_grid;
+    res.stage_ms[gst::kStageIndices] = 0.f;
+    res.stage_ms[gst::kStageAssemble] = fused_ms;
+  } else {
+    best.fused_waves = 0;
+    grids[gst::kStageIndices] = sep_grid[0];
+    grids[gst::kStageAssemble] = sep_grid[1];
+    res.stage_ms[gst::kStageIndices] = sep_ms[0];
+    res.stage_ms[gst::kStageAssemble] = sep_ms[1];
   }
 
   b->stage = best;
@@ -821,6 +879,7 @@ int gst_batch_autotune(gst_batch *b, gst_stream stream, uint32_t reps, gst_tune_
   res.ans_groups_per_wave = best.ans_groups_per_wave;
   res.ans_waves_per_group = best.ans_waves;
   res.wavelet_waves_per_group = best.wavelet_waves;
+  res.assembly_waves_per_group = best.fused_waves;
   res.grid_ans = grids[gst::kStageAns];
   res.grid_indices = grids[gst::kStageIndices];
   res.grid_wavelet = grids[gst::kStageWavelet];
@@ -939,13 +998,15 @@ int StageLaunch(uint32_t stage, const gst_header *hdr, const uint8_t *d_freqs, c
   d.planes = const_cast<int8_t *>(d_planes);
   d.indices = const_cast<int32_t *>(d_indices);
   d.group_sums = const_cast<int32_t *>(d_group_sums);
+  d.lane_sums = nullptr;  // the per-stage buffers hold no room for lane totals
   KernelArgs a;
   memset(&a, 0, sizeof(a));
   a.single = d;
   a.err = d_err;
   a.tables = const_cast<uint32_t *>(d_tables);
   a.table_status = const_cast<uint32_t *>(d_tables) + 4 * gst::kTableSize;
-  const StageConfig cfg = DefaultStageConfig(rgb);
+  StageConfig cfg = DefaultStageConfig(rgb);
+  cfg.fused_waves = 0;  // each stage is its own launch here
   const uint32_t grid = gst::SinglePassGrid(stage, gst::StageUnitsOf(d, stage, GroupsPerUnit(cfg)), cfg);
   GST_HIP(gst::LaunchStage(stage, a, grid, cfg, s));
   return GST_OK;

@@ -51,6 +51,7 @@ struct ImageDesc {
   int8_t *planes;
   int32_t *indices;
   int32_t *group_sums;   // one per index-stream group
+  int32_t *lane_sums;    // one per index-stream lane (nv / 256), or null
   uint8_t *out;
 };
 
@@ -100,6 +101,8 @@ struct StageConfig {
   uint32_t ans_waves;            // waves per rANS work-group (1, 2, 4, 8)
   uint32_t ans_groups_per_wave;  // 1 or 2
   uint32_t wavelet_waves;        // waves per wavelet work-group (1, 2, 4); a wave takes 2 units
+  uint32_t fused_waves;          // 0: separate index scan and assembly; else waves per
+                                 // work-group of the fused scan + DXT1 assembly (1, 2, 4)
   uint32_t rgb;                  // 1: RGB8 assembly
 };
 
@@ -109,6 +112,8 @@ inline uint32_t SinglePassGrid(uint32_t stage, uint32_t units, const StageConfig
     const uint32_t per = 2 * c.wavelet_waves;
     return (units + per - 1) / per;
   }
+  if (c.fused_waves && stage == kStageIndices) return 0;  // folded into the assembly
+  if (c.fused_waves && stage == kStageAssemble) return (units + c.fused_waves - 1) / c.fused_waves;
   return units;
 }
 

@@ -144,7 +144,8 @@ __global__ __launch_bounds__(256) void BuildTableKernel(KernelArgs a) {
 // group's first byte raises kErrStreamOffsets.  Each lane packs 16 symbols into four
 // registers and writes them with one 16-byte store, in the reference's output order
 // (lane j, step i of the group at j*256 + 255 - i).  Index-stream groups also emit
-// the sum of (symbol - 128), the per-group total the index scan starts from.
+// the sum of (symbol - 128) per group and per lane, the totals the index scan
+// starts from.
 constexpr uint32_t kChunkWords = 512;
 constexpr uint32_t kRingWords = 2 * kChunkWords;
 
@@ -171,6 +172,7 @@ __device__ __forceinline__ void DecodeHalfGroups(const uint2 *table, uint16_t *r
                                                  __amdgpu_buffer_rsrc_t rs, uint32_t lane, bool active,
                                                  uint32_t x, uint32_t pA, uint32_t pB, uint32_t loA,
                                                  uint32_t loB, uint8_t *outp, int32_t *group_sum,
+                                                 int32_t *lane_sum,
                                                  uint32_t *err) {
   const uint32_t j = lane & 31;
   uint32_t upper = lane < 32 ? 0u : 0xFFFFFFFFu;
@@ -247,6 +249,7 @@ __device__ __forceinline__ void DecodeHalfGroups(const uint2 *table, uint16_t *r
   const bool under = lane < 32 ? (pA < loA) : (pB < loB);
   if (active && under && j == 0) atomicOr(err, kErrStreamOffsets);
   if (kSum) {
+    if (active && lane_sum) lane_sum[j] = sum;
     sum = WaveSumWidth32(sum);
     if (active && j == 0) *group_sum = sum;
   }
@@ -327,10 +330,11 @@ __global__ __launch_bounds__(WAVES * 64) void AnsDecodeKernel(KernelArgs a) {
     uint16_t *ring_half = ring[wave][half];
     if (k == 3)
       DecodeHalfGroups<true, GPW>(table, ring_half, rs, lane, active, x, pA, pB, loA, loB, outp,
-                             d.group_sums + g, a.err);
+                                  d.group_sums + g,
+                                  d.lane_sums ? d.lane_sums + g * kLanesPerGroup : nullptr, a.err);
     else
       DecodeHalfGroups<false, GPW>(table, ring_half, rs, lane, active, x, pA, pB, loA, loB, outp, nullptr,
-                              a.err);
+                                   nullptr, a.err);
   }
 }
 
@@ -621,6 +625,82 @@ __global__ __launch_bounds__(256) void AssembleDxtKernel(KernelArgs a) {
   }
 }
 
+// Index scan fused into DXT1 assembly: one wave per 1024-block unit, which covers
+// four 256-block segments of the index stream (one rANS lane each).  The value
+// before the unit is the sum of the earlier groups' totals and the earlier lanes'
+// totals of its group, both written by the rANS stage; a segment is a 4-per-lane
+// register prefix plus a wave scan.  The palette indices never go to memory, so a
+// block costs 1 diff byte instead of 1 + 4 + 4 bytes of index traffic.  Results are
+// those of DecodeIndicesKernel followed by AssembleDxtKernel.
+__device__ __forceinline__ int32_t WaveSum64(int32_t v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+template <uint32_t WAVES>
+__global__ __launch_bounds__(WAVES * 64) void AssembleFusedKernel(KernelArgs a) {
+  constexpr uint32_t kSegments = kAssembleBlocksPerUnit / kSymsPerLane;  // 4
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t wave = Uniform(threadIdx.x >> 6);
+  const uint32_t total = StageTotal(a, kStageAssemble, 1);
+  for (uint32_t u = blockIdx.x * WAVES + wave; u < total; u += gridDim.x * WAVES) {
+    const Located loc = Locate(a, kStageAssemble, u);
+    const ImageDesc d = LoadDesc(a, loc.img);
+    const uint32_t nv = d.nv;
+    const uint32_t b0 = loc.local * kAssembleBlocksPerUnit;
+    const uint32_t g = b0 / kGroupSyms;
+    const uint32_t seg0 = (b0 % kGroupSyms) / kSymsPerLane;
+    int32_t base = 0;
+    for (uint32_t i = lane; i < g; i += 64) base += d.group_sums[i];
+    if (lane < seg0) base += d.lane_sums[g * kLanesPerGroup + lane];
+    base = WaveSum64(base);
+    const uint8_t *diffs = d.symbols + SymbolOffset(d, 3);
+    const uint32_t *pal = reinterpret_cast<const uint32_t *>(d.symbols + SymbolOffset(d, 2));
+    const uint32_t words = d.palette_bytes / 4;
+    bool bad = false;
+#pragma unroll
+    for (uint32_t seg = 0; seg < kSegments; ++seg) {
+      const uint32_t blk = b0 + seg * kSymsPerLane + 4 * lane;
+      const uint32_t dw = *reinterpret_cast<const uint32_t *>(diffs + blk);
+      uint32_t pw[6];
+#pragma unroll
+      for (int p = 0; p < 6; ++p)
+        pw[p] = *reinterpret_cast<const uint32_t *>(d.planes + static_cast<size_t>(p) * nv + blk);
+      int32_t run[4];
+      int32_t acc = 0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        acc += static_cast<int32_t>((dw >> (8 * q)) & 0xFFu) - 128;
+        run[q] = acc;
+      }
+      int32_t incl = acc;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const int32_t n = __shfl_up(incl, off);
+        if (lane >= static_cast<uint32_t>(off)) incl += n;
+      }
+      const int32_t before = base + incl - acc;
+      base += __shfl(incl, 63);
+      uint32_t o[8];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const uint32_t ep1 = Rgb565(SByte(pw[0], q), SByte(pw[2], q), SByte(pw[3], q));
+        const uint32_t ep2 = Rgb565(SByte(pw[1], q), SByte(pw[4], q), SByte(pw[5], q));
+        const uint32_t pi = static_cast<uint32_t>(before + run[q]);
+        uint32_t word = 0;
+        if (pi < words) word = pal[pi]; else bad = true;
+        o[2 * q] = ep1 | (ep2 << 16);
+        o[2 * q + 1] = word;
+      }
+      uint4 *dst = reinterpret_cast<uint4 *>(d.out + 8 * static_cast<size_t>(blk));
+      dst[0] = make_uint4(o[0], o[1], o[2], o[3]);
+      dst[1] = make_uint4(o[4], o[5], o[6], o[7]);
+    }
+    if (bad) atomicOr(a.err, kErrPaletteIndex);
+  }
+}
+
 __global__ __launch_bounds__(256) void AssembleRgbKernel(KernelArgs a) {
   const uint32_t total = StageTotal(a, kStageAssemble, 1);
   for (uint32_t u = blockIdx.x; u < total; u += gridDim.x) {
@@ -710,6 +790,7 @@ hipError_t LaunchStage(uint32_t stage, const KernelArgs &args, uint32_t grid,
       return hipErrorInvalidValue;
     }
     case kStageIndices:
+      if (cfg.fused_waves) return hipSuccess;  // the assembly scans the indices itself
       hipLaunchKernelGGL(DecodeIndicesKernel, dim3(grid), dim3(256), 0, stream, args);
       return hipGetLastError();
     case kStageWavelet:
@@ -723,10 +804,20 @@ hipError_t LaunchStage(uint32_t stage, const KernelArgs &args, uint32_t grid,
         return hipErrorInvalidValue;
       return hipGetLastError();
     case kStageAssemble:
-      if (cfg.rgb)
+      if (cfg.rgb) {
+        if (cfg.fused_waves) return hipErrorInvalidValue;  // the fused scan writes DXT1 only
         hipLaunchKernelGGL(AssembleRgbKernel, dim3(grid), dim3(256), 0, stream, args);
-      else
+      } else if (cfg.fused_waves == 1) {
+        hipLaunchKernelGGL(AssembleFusedKernel<1>, dim3(grid), dim3(64), 0, stream, args);
+      } else if (cfg.fused_waves == 2) {
+        hipLaunchKernelGGL(AssembleFusedKernel<2>, dim3(grid), dim3(128), 0, stream, args);
+      } else if (cfg.fused_waves == 4) {
+        hipLaunchKernelGGL(AssembleFusedKernel<4>, dim3(grid), dim3(256), 0, stream, args);
+      } else if (cfg.fused_waves == 0) {
         hipLaunchKernelGGL(AssembleDxtKernel, dim3(grid), dim3(256), 0, stream, args);
+      } else {
+        return hipErrorInvalidValue;
+      }
       return hipGetLastError();
     default:
       return hipErrorInvalidValue;

@@ -131,7 +131,8 @@ int gst_pack_staging(const uint8_t *const *files, const size_t *lens, uint32_t n
  * reference's ANS output layout [Y 2nv][chroma 4nv][palette P][indices nv]
  * (codec/decoder.cpp:441-461, ans/ans_decode.cl:154-155); d_group_sums nv/8192
  * int32; d_planes 6*nv int8 raster planes Y1,Y2,Co1,Cg1,Co2,Cg2; d_indices nv int32;
- * d_err one u32 of GST_DEVERR_* bits (cleared by the build_table launch). */
+ * d_err one u32 of GST_DEVERR_* bits, which the stages only OR into: the caller
+ * zeroes it before the first stage and reads it after the last. */
 
 /* ans/build_table.cl:12-83 (codec/decoder.cpp:164-198): 4 tables from 4 x 256 u16. */
 int gst_stage_build_table(const void *d_freqs, uint32_t *d_tables, uint32_t *d_err,
@@ -177,6 +178,9 @@ typedef struct gst_batch_config {
   uint32_t rgb_output;        /* 1: RGB8 assembly instead of DXT1 */
   uint32_t wavelet_waves_per_group; /* waves per work-group of the wavelet kernel, each
                                        transforming 4 tiles at a time: 1, 2 or 4; 0 = 1 */
+  uint32_t assembly_waves_per_group; /* waves per work-group of the DXT1 assembly with the
+                                        index scan folded in: 1, 2 or 4; 0 = 4 */
+  uint32_t separate_index_scan;     /* 1: index scan and assembly as two launches */
 } gst_batch_config;
 
 int gst_batch_create(const gst_batch_config *cfg, gst_batch **out);
@@ -202,6 +206,7 @@ typedef struct gst_tune_result {
   uint32_t ans_groups_per_wave;      /* 1: 32 streams per wavefront, 2: 64 */
   uint32_t ans_waves_per_group;
   uint32_t wavelet_waves_per_group;  /* 4 tiles per wave */
+  uint32_t assembly_waves_per_group; /* 0: separate index scan and assembly */
   uint32_t grid_ans, grid_indices, grid_wavelet, grid_assemble;
   uint32_t candidates;               /* shapes timed */
   float stage_ms[5];                 /* median time of the chosen shape per stage */
@@ -210,8 +215,9 @@ typedef struct gst_tune_result {
 /* No reference counterpart.  Times every launch shape of every stage on the bound
  * batch (median of `reps` launches each, 0 = 5) and keeps the fastest per stage:
  * rANS streams per wavefront {32, 64} x waves per work-group {1, 2, 4, 8} x grid,
- * wavelet waves per work-group {1, 2, 4} x grid, and the grids of the index scan and
- * the assembly.  The bound images are decoded along the way.  Drops a captured graph
+ * wavelet waves per work-group {1, 2, 4} x grid, and the index scan + assembly either
+ * fused (waves per work-group {1, 2, 4} x grid) or as two launches (grid each).  The
+ * bound images are decoded along the way.  Drops a captured graph
  * (capture again afterwards).  Blocking. */
 int gst_batch_autotune(gst_batch *b, gst_stream stream, uint32_t reps, gst_tune_result *out);
 /* Device bytes the plan holds (descriptors + tables + scratch). */

@@ -46,13 +46,14 @@ def test_uneven_batch_matches_oracle(device):
         b.close()
 
 
-@pytest.mark.parametrize("gpw,waves,wwaves", [(2, 4, 1), (1, 4, 2), (2, 8, 4), (2, 1, 1), (1, 2, 4)])
-def test_launch_shapes_agree(device, gpw, waves, wwaves):
+@pytest.mark.parametrize("gpw,waves,wwaves,awaves,separate", [
+    (2, 4, 1, 4, False), (1, 4, 2, 1, False), (2, 8, 4, 2, False), (2, 1, 1, 0, True), (1, 2, 4, 0, True)])
+def test_launch_shapes_agree(device, gpw, waves, wwaves, awaves, separate):
     files, dxts = encode.make_textures(MIXED[:4])
     blocks, pal = _capacity(files)
     b = gst_amd.Batch(len(files), blocks, pal, groups_per_wave=gpw, waves_per_group=waves,
                       grids={"ans_decode": 64, "inv_wavelet": 128, "decode_indices": 16, "assemble": 32},
-                      wavelet_waves=wwaves)
+                      wavelet_waves=wwaves, assembly_waves=awaves, separate_index_scan=separate)
     try:
         _, _, d_outs = _bind(device, b, files)
         b.decode(device.stream)

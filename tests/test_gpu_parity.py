@@ -199,10 +199,16 @@ def test_bad_group_offsets_are_reported(device):
         assert e.value.code == -3
 
 
-def test_garbled_renormalisation_words_stay_in_bounds(device):
+@pytest.mark.parametrize("stream", ["palette", "indices"])
+def test_garbled_renormalisation_words_stay_in_bounds(device, stream):
+    """Noise in a stream's words decodes to other symbols; palette indices that leave
+    the palette raise the device error bit instead of reading past it."""
     gtc = load_golden("ref_c0")["gtc"].tobytes()
     h = gst_amd.ParseHeader(gtc)
-    start = 28 + 2048 + h.y_cmp_sz + h.chroma_cmp_sz + 4 * (h.palette_bytes // 8192)
+    if stream == "palette":
+        start = 28 + 2048 + h.y_cmp_sz + h.chroma_cmp_sz + 4 * (h.palette_bytes // 8192)
+    else:
+        start = 28 + 2048 + h.y_cmp_sz + h.chroma_cmp_sz + h.palette_sz + 4 * (h.num_blocks // 8192)
     rng = np.random.default_rng(5)
     noise = rng.integers(0, 256, 200, dtype=np.uint8).tobytes()
     bad = _corrupt(gtc, lambda b: b.__setitem__(slice(start, start + 200), noise))
