@@ -134,7 +134,10 @@ def test_autotune_keeps_results_bit_exact(device):
         assert tuned["streams_per_wavefront"] in (32, 64)
         assert tuned["wavelet_waves_per_group"] in (1, 2, 4)
         assert tuned["candidates"] >= 7 * 5 + 3 * 5 + 2 * 5
-        assert all(v > 0 for v in tuned["stage_ms"].values())
+        fused = tuned["assembly_waves_per_group"] > 0
+        for k, v in tuned["stage_ms"].items():
+            # A fused index scan has no launch of its own.
+            assert v == 0 if (fused and k == "decode_indices") else v > 0, (k, v)
         for o in outs:
             o.zero_()
         b.decode(device.stream)
