@@ -198,52 +198,68 @@ __device__ __forceinline__ void DecodeHalfGroups(const uint2 *table, uint16_t *r
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 
   int32_t sum = 0;
-  for (uint32_t blk = 0; blk < kSymsPerLane / 16; ++blk) {
-    uint32_t acc0 = 0, acc1 = 0, acc2 = 0, acc3 = 0;
+  // 64 steps fill one 64-byte sector of the lane's output; whole sectors leave in four
+  // back-to-back 16-byte stores (smaller pieces of a lane's segment, 256 B apart
+  // across lanes, are evicted half-written and written back several times).
+  for (uint32_t sector = 0; sector < kSymsPerLane / 64; ++sector) {
+    uint32_t acc[16];
 #pragma unroll
-    for (uint32_t s = 0; s < 16; ++s) {
-      const uint2 e = table[x & (kTableSize - 1)];
-      x = __umul24(x >> kTableLog, e.x) + e.y;  // (x >> 11) * f + slot - cum (low 24 bits of e.x = f)
-      const uint64_t m = __builtin_amdgcn_uicmp(x, kStateL, 36 /* ULT */) & active_mask;
-      const uint32_t mlo = static_cast<uint32_t>(m), mhi = static_cast<uint32_t>(m >> 32);
-      const uint32_t tlo = __popc(mlo), thi = __popc(mhi);
-      pA -= tlo;
-      pB -= thi;
-      if (active && x < kStateL) {
-        // Word index p_half + rank: mbcnt over both halves counts the lower half's
-        // bits for upper lanes, so upper lanes start from pB - tlo.
-        const uint32_t b = pA + (upper & (pB - tlo - pA));
-        const uint32_t widx = __builtin_amdgcn_mbcnt_hi(mhi, __builtin_amdgcn_mbcnt_lo(mlo, b));
-        x = (x << 16) | ring_half[widx & (kRingWords - 1)];
+    for (uint32_t i = 0; i < 16; ++i) acc[i] = 0;
+#pragma unroll
+    for (uint32_t blk = 0; blk < 4; ++blk) {
+#pragma unroll
+      for (uint32_t s = 0; s < 16; ++s) {
+        const uint2 e = table[x & (kTableSize - 1)];
+        x = __umul24(x >> kTableLog, e.x) + e.y;  // (x >> 11) * f + slot - cum (low 24 bits of e.x = f)
+        const uint64_t m = __builtin_amdgcn_uicmp(x, kStateL, 36 /* ULT */) & active_mask;
+        const uint32_t mlo = static_cast<uint32_t>(m), mhi = static_cast<uint32_t>(m >> 32);
+        const uint32_t tlo = __popc(mlo), thi = __popc(mhi);
+        pA -= tlo;
+        pB -= thi;
+        if (active && x < kStateL) {
+          // Word index p_half + rank: mbcnt over both halves counts the lower half's
+          // bits for upper lanes, so upper lanes start from pB - tlo.
+          const uint32_t b = pA + (upper & (pB - tlo - pA));
+          const uint32_t widx = __builtin_amdgcn_mbcnt_hi(mhi, __builtin_amdgcn_mbcnt_lo(mlo, b));
+          x = (x << 16) | ring_half[widx & (kRingWords - 1)];
+        }
+        // Step 16*blk + s lands at byte 63 - 16*blk - s of the sector.  The symbol
+        // (byte 3 of e.x) goes in with v_perm: selector codes 4-7 take bytes of the
+        // first operand, 0-3 bytes of the second.
+        const uint32_t pos = 63 - 16 * blk - s;
+        const uint32_t sel = (0x03020100u & ~(0xFFu << (8 * (pos & 3)))) | (7u << (8 * (pos & 3)));
+        acc[pos >> 2] = __builtin_amdgcn_perm(e.x, acc[pos >> 2], sel);
+        asm volatile("" : "+v"(acc[pos >> 2]));  // insert now: keeps 64 symbols out of VGPRs
       }
-      // Symbol (byte 3 of e.x) into byte `pos` of the 16-byte chunk: v_perm selector
-      // codes 4-7 take bytes of the first operand, 0-3 bytes of the second.
-      const uint32_t pos = 15 - s;
-      const uint32_t sel = (0x03020100u & ~(0xFFu << (8 * (pos & 3)))) | (7u << (8 * (pos & 3)));
-      if ((pos >> 2) == 0) acc0 = __builtin_amdgcn_perm(e.x, acc0, sel);
-      else if ((pos >> 2) == 1) acc1 = __builtin_amdgcn_perm(e.x, acc1, sel);
-      else if ((pos >> 2) == 2) acc2 = __builtin_amdgcn_perm(e.x, acc2, sel);
-      else acc3 = __builtin_amdgcn_perm(e.x, acc3, sel);
-      if (kSum) sum += static_cast<int32_t>(e.x >> 24) - 128;
+      // Ring refill at the 16-step boundary: the pointer moved at most 512 words, so it
+      // crossed at most one chunk boundary, and the next 16 steps read only chunks
+      // top and top-1 (top = chunk of pointer - 1).
+      const bool crossA = pA < topA * kChunkWords, crossB = pB < topB * kChunkWords;
+      if (crossA || crossB) {
+        if (lane < 32 ? crossA : crossB) {
+          const uint32_t t = lane < 32 ? topA : topB;
+          StoreChunk(ring_half, t - 2, j, pend0, pend1);
+          LoadChunk(rs, t - 3, j, pend0, pend1);
+        }
+        topA -= crossA ? 1u : 0u;
+        topB -= crossB ? 1u : 0u;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      }
     }
-    if (active)
-      *reinterpret_cast<uint4 *>(outp + (kSymsPerLane - 16) - 16 * blk) =
-          make_uint4(acc0, acc1, acc2, acc3);
-    // Ring refill at the 16-step boundary: the pointer moved at most 512 words, so it
-    // crossed at most one chunk boundary, and the next 16 steps read only chunks
-    // top and top-1 (top = chunk of pointer - 1).
-    const bool crossA = pA < topA * kChunkWords, crossB = pB < topB * kChunkWords;
-    if (crossA || crossB) {
-      if (lane < 32 ? crossA : crossB) {
-        const uint32_t t = lane < 32 ? topA : topB;
-        StoreChunk(ring_half, t - 2, j, pend0, pend1);
-        LoadChunk(rs, t - 3, j, pend0, pend1);
-      }
-      topA -= crossA ? 1u : 0u;
-      topB -= crossB ? 1u : 0u;
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (kSum) {
+      // Byte sum of the sector (v_sad_u8 against zero), less the 64 biases.
+      uint32_t bytes = 0;
+#pragma unroll
+      for (uint32_t i = 0; i < 16; ++i) bytes = __builtin_amdgcn_sad_u8(acc[i], 0u, bytes);
+      sum += static_cast<int32_t>(bytes) - 64 * 128;
+    }
+    if (active) {
+      uint4 *dst = reinterpret_cast<uint4 *>(outp + (kSymsPerLane - 64) - 64 * sector);
+#pragma unroll
+      for (uint32_t i = 0; i < 4; ++i)
+        dst[i] = make_uint4(acc[4 * i], acc[4 * i + 1], acc[4 * i + 2], acc[4 * i + 3]);
     }
   }
   const bool under = lane < 32 ? (pA < loA) : (pB < loB);
