@@ -123,8 +123,9 @@ __global__ __launch_bounds__(256) void BuildTableKernel(KernelArgs a) {
 // Stage 2: interleaved rANS decode (ans/ans_decode.cl:97-157).
 //
 // A work-group of WAVES wave64s decodes GPW*WAVES consecutive groups of one stream
-// with that stream's table resident in LDS, widened to 8 bytes per slot
-// {f | symbol << 24, slot - cum} so the state update is one v_mad_u32_u24.  With
+// with that stream's table resident in LDS as two arrays, u32 {f | symbol << 24} and
+// u16 {slot - cum}, so the state update is one v_mad_u32_u24 (12 KiB per table: with
+// the rings below a 4-wave work-group takes 20 KiB, and 7 of them fit a CU).  With
 // GPW = 2 a wave carries two 32-stream groups, one per half-wave; each lane holds
 // one stream's state.  The reference's local-memory renormalisation mask
 // (atomic_or / popcount / atomic_and between barriers) becomes one ballot: a lane
@@ -132,43 +133,40 @@ __global__ __launch_bounds__(256) void BuildTableKernel(KernelArgs a) {
 // coming from mbcnt over the 64-bit ballot, and each half-wave's pointer (a scalar)
 // moves down by its total.
 //
-// The words are read from an LDS ring of two 512-word chunks per half-wave, so the
+// The words are read from an LDS ring of two 256-word chunks per half-wave, so the
 // serial chain of a step only touches LDS.  Chunk c covers stream words
-// [512c, 512c + 512) and lives at ring slot c & 1.  With `top` the chunk holding the
-// half's pointer - 1, chunks top and top-1 are resident and chunk top-2 is in flight
-// in registers.  A step moves the pointer by at most 32 words, so 16 steps read
-// only chunks top and top-1; every 16 steps, if the pointer has dropped below
-// 512*top, chunk top-2 is written over top's slot and chunk top-3 is requested,
-// which then has at least 16 steps to arrive.  Stream loads go through a buffer resource bounded by the stream
-// length (corrupt offsets read zeros, never fault); a pointer that ends below the
-// group's first byte raises kErrStreamOffsets.  Each lane packs 16 symbols into four
-// registers and writes them with one 16-byte store, in the reference's output order
-// (lane j, step i of the group at j*256 + 255 - i).  Index-stream groups also emit
+// [256c, 256c + 256) and lives at ring slot c & 1.  With `top` the chunk holding the
+// half's pointer - 1, chunks top and top-1 are resident and chunks top-2 and top-3
+// are in flight in registers.  A step moves the pointer by at most 32 words, so 8
+// steps read only chunks top and top-1; every 8 steps, if the pointer has dropped
+// below 256*top, chunk top-2 is written over top's slot, top-3 moves up and top-4 is
+// requested, which then has at least 16 steps to arrive.  Stream loads go through a
+// buffer resource bounded by the stream length (corrupt offsets read zeros, never
+// fault); a pointer that ends below the group's first byte raises kErrStreamOffsets.
+// Each lane packs 64 symbols into 16 registers and writes them as one 64-byte sector,
+// in the reference's output order (lane j, step i of the group at j*256 + 255 - i).
+// Index-stream groups also emit
 // the sum of (symbol - 128) per group and per lane, the totals the index scan
 // starts from.
-constexpr uint32_t kChunkWords = 512;
+constexpr uint32_t kChunkWords = 256;
 constexpr uint32_t kRingWords = 2 * kChunkWords;
+constexpr uint32_t kRefillSteps = kChunkWords / kLanesPerGroup;  // 8
 
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
-// Lane j's 32 bytes of chunk c (words 16j .. 16j+15); chunks below the stream
-// start wrap to huge offsets and read zeros.
-__device__ __forceinline__ void LoadChunk(__amdgpu_buffer_rsrc_t rs, uint32_t c, uint32_t j, u32x4 &a,
-                                          u32x4 &b) {
-  const int off = static_cast<int>((c * kChunkWords + 16 * j) * 2);
-  a = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0);
-  b = __builtin_amdgcn_raw_buffer_load_b128(rs, off + 16, 0, 0);
+// Lane j's 16 bytes of chunk c (words 8j .. 8j+7); chunks below the stream start
+// wrap to huge offsets and read zeros.
+__device__ __forceinline__ u32x4 LoadChunk(__amdgpu_buffer_rsrc_t rs, uint32_t c, uint32_t j) {
+  return __builtin_amdgcn_raw_buffer_load_b128(rs, static_cast<int>((c * kChunkWords + 8 * j) * 2), 0, 0);
 }
 
-__device__ __forceinline__ void StoreChunk(uint16_t *ring, uint32_t c, uint32_t j, const u32x4 &a,
-                                           const u32x4 &b) {
-  u32x4 *dst = reinterpret_cast<u32x4 *>(ring + ((c * kChunkWords + 16 * j) & (kRingWords - 1)));
-  dst[0] = a;
-  dst[1] = b;
+__device__ __forceinline__ void StoreChunk(uint16_t *ring, uint32_t c, uint32_t j, const u32x4 &a) {
+  *reinterpret_cast<u32x4 *>(ring + ((c * kChunkWords + 8 * j) & (kRingWords - 1))) = a;
 }
 
 template <bool kSum, uint32_t GPW>
-__device__ __forceinline__ void DecodeHalfGroups(const uint2 *table, uint16_t *ring_half,
+__device__ __forceinline__ void DecodeHalfGroups(const uint32_t *tab_fs, const uint16_t *tab_bias,
+                                                 uint16_t *ring_half,
                                                  __amdgpu_buffer_rsrc_t rs, uint32_t lane, bool active,
                                                  uint32_t x, uint32_t pA, uint32_t pB, uint32_t loA,
                                                  uint32_t loB, uint8_t *outp, int32_t *group_sum,
@@ -178,21 +176,20 @@ __device__ __forceinline__ void DecodeHalfGroups(const uint2 *table, uint16_t *r
   uint32_t upper = lane < 32 ? 0u : 0xFFFFFFFFu;
   asm volatile("" : "+v"(upper));  // keep the half-wave mask an opaque VGPR (one v_and below)
   const uint64_t active_mask = __builtin_amdgcn_ballot_w64(active);
-  // Ring prologue: chunks top and top-1 resident, top-2 in flight.
+  // Ring prologue: chunks top and top-1 resident, top-2 and top-3 in flight.
   uint32_t topA = pA ? (pA - 1) / kChunkWords : 0;
   uint32_t topB = pB ? (pB - 1) / kChunkWords : 0;
   const uint32_t top = lane < 32 ? topA : topB;
   {
-    u32x4 a0, a1, b0, b1;
-    LoadChunk(rs, top, j, a0, a1);
-    LoadChunk(rs, top - 1, j, b0, b1);
+    const u32x4 c0 = LoadChunk(rs, top, j);
+    const u32x4 c1 = LoadChunk(rs, top - 1, j);
     if (GPW == 2 || lane < 32) {  // one group per wave: the idle upper half shares the ring
-      StoreChunk(ring_half, top, j, a0, a1);
-      StoreChunk(ring_half, top - 1, j, b0, b1);
+      StoreChunk(ring_half, top, j, c0);
+      StoreChunk(ring_half, top - 1, j, c1);
     }
   }
-  u32x4 pend0, pend1;
-  LoadChunk(rs, top - 2, j, pend0, pend1);
+  u32x4 pend0 = LoadChunk(rs, top - 2, j);
+  u32x4 pend1 = LoadChunk(rs, top - 3, j);
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -206,11 +203,13 @@ __device__ __forceinline__ void DecodeHalfGroups(const uint2 *table, uint16_t *r
 #pragma unroll
     for (uint32_t i = 0; i < 16; ++i) acc[i] = 0;
 #pragma unroll
-    for (uint32_t blk = 0; blk < 4; ++blk) {
+    for (uint32_t blk = 0; blk < 64 / kRefillSteps; ++blk) {
 #pragma unroll
-      for (uint32_t s = 0; s < 16; ++s) {
-        const uint2 e = table[x & (kTableSize - 1)];
-        x = __umul24(x >> kTableLog, e.x) + e.y;  // (x >> 11) * f + slot - cum (low 24 bits of e.x = f)
+      for (uint32_t s = 0; s < kRefillSteps; ++s) {
+        const uint32_t slot = x & (kTableSize - 1);
+        const uint32_t fs = tab_fs[slot];
+        // (x >> 11) * f + slot - cum; the low 24 bits of fs are f.
+        x = __umul24(x >> kTableLog, fs) + tab_bias[slot];
         const uint64_t m = __builtin_amdgcn_uicmp(x, kStateL, 36 /* ULT */) & active_mask;
         const uint32_t mlo = static_cast<uint32_t>(m), mhi = static_cast<uint32_t>(m >> 32);
         const uint32_t tlo = __popc(mlo), thi = __popc(mhi);
@@ -223,23 +222,24 @@ __device__ __forceinline__ void DecodeHalfGroups(const uint2 *table, uint16_t *r
           const uint32_t widx = __builtin_amdgcn_mbcnt_hi(mhi, __builtin_amdgcn_mbcnt_lo(mlo, b));
           x = (x << 16) | ring_half[widx & (kRingWords - 1)];
         }
-        // Step 16*blk + s lands at byte 63 - 16*blk - s of the sector.  The symbol
-        // (byte 3 of e.x) goes in with v_perm: selector codes 4-7 take bytes of the
-        // first operand, 0-3 bytes of the second.
-        const uint32_t pos = 63 - 16 * blk - s;
+        // Step 8*blk + s lands at byte 63 - 8*blk - s of the sector.  The symbol (byte
+        // 3 of fs) goes in with v_perm: selector codes 4-7 take bytes of the first
+        // operand, 0-3 bytes of the second.
+        const uint32_t pos = 63 - kRefillSteps * blk - s;
         const uint32_t sel = (0x03020100u & ~(0xFFu << (8 * (pos & 3)))) | (7u << (8 * (pos & 3)));
-        acc[pos >> 2] = __builtin_amdgcn_perm(e.x, acc[pos >> 2], sel);
+        acc[pos >> 2] = __builtin_amdgcn_perm(fs, acc[pos >> 2], sel);
         asm volatile("" : "+v"(acc[pos >> 2]));  // insert now: keeps 64 symbols out of VGPRs
       }
-      // Ring refill at the 16-step boundary: the pointer moved at most 512 words, so it
-      // crossed at most one chunk boundary, and the next 16 steps read only chunks
+      // Ring refill at the 8-step boundary: the pointer moved at most 256 words, so it
+      // crossed at most one chunk boundary, and the next 8 steps read only chunks
       // top and top-1 (top = chunk of pointer - 1).
       const bool crossA = pA < topA * kChunkWords, crossB = pB < topB * kChunkWords;
       if (crossA || crossB) {
         if (lane < 32 ? crossA : crossB) {
           const uint32_t t = lane < 32 ? topA : topB;
-          StoreChunk(ring_half, t - 2, j, pend0, pend1);
-          LoadChunk(rs, t - 3, j, pend0, pend1);
+          StoreChunk(ring_half, t - 2, j, pend0);
+          pend0 = pend1;
+          pend1 = LoadChunk(rs, t - 4, j);
         }
         topA -= crossA ? 1u : 0u;
         topB -= crossB ? 1u : 0u;
@@ -286,7 +286,8 @@ __device__ __forceinline__ const uint8_t *Uniform(const uint8_t *p) {
 
 template <uint32_t WAVES, uint32_t GPW>
 __global__ __launch_bounds__(WAVES * 64) void AnsDecodeKernel(KernelArgs a) {
-  __shared__ uint2 table[kTableSize];
+  __shared__ uint32_t tab_fs[kTableSize];
+  __shared__ uint16_t tab_bias[kTableSize];
   __shared__ uint16_t ring[WAVES][2][kRingWords];
   constexpr uint32_t kGroupsPerUnit = WAVES * GPW;
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -310,7 +311,8 @@ __global__ __launch_bounds__(WAVES * 64) void AnsDecodeKernel(KernelArgs a) {
       const uint32_t *src = a.tables + static_cast<size_t>(tix) * kTableSize;
       for (uint32_t i = threadIdx.x; i < kTableSize; i += WAVES * 64) {
         const uint32_t e = src[i];
-        table[i] = make_uint2((e & 0xFFFu) | (e & 0xFF000000u), (e >> 12) & 0x7FFu);
+        tab_fs[i] = (e & 0xFFFu) | (e & 0xFF000000u);
+        tab_bias[i] = static_cast<uint16_t>((e >> 12) & 0x7FFu);
       }
     }
     const bool table_ok = a.table_status[tix] != 0;
@@ -345,11 +347,12 @@ __global__ __launch_bounds__(WAVES * 64) void AnsDecodeKernel(KernelArgs a) {
     uint8_t *outp = d.symbols + SymbolOffset(d, k) + static_cast<size_t>(g) * kGroupSyms + j * kSymsPerLane;
     uint16_t *ring_half = ring[wave][half];
     if (k == 3)
-      DecodeHalfGroups<true, GPW>(table, ring_half, rs, lane, active, x, pA, pB, loA, loB, outp,
+      DecodeHalfGroups<true, GPW>(tab_fs, tab_bias, ring_half, rs, lane, active, x, pA, pB, loA, loB, outp,
                                   d.group_sums + g,
                                   d.lane_sums ? d.lane_sums + g * kLanesPerGroup : nullptr, a.err);
     else
-      DecodeHalfGroups<false, GPW>(table, ring_half, rs, lane, active, x, pA, pB, loA, loB, outp, nullptr,
+      DecodeHalfGroups<false, GPW>(tab_fs, tab_bias, ring_half, rs, lane, active, x, pA, pB, loA, loB, outp,
+                                   nullptr,
                                    nullptr, a.err);
   }
 }
