@@ -138,11 +138,9 @@ def run_batched(gpu, files, dxts, steps, warmup, tune=True):
     for _ in range(warmup):
         b.launch_graph(gpu.stream)
     ms = gpu.timed(lambda: b.launch_graph(gpu.stream), steps)
-    stage = {k: 0.0 for k in gst_amd.STAGES}
-    reps = 5
-    for _ in range(reps):
-        for k, v in b.decode_timed(gpu.stream).items():
-            stage[k] += v / reps
+    # Per-stage kernel time inside the same pipeline: hipEvents recorded around each
+    # stage in a copy of the captured graph, on the decode stream, mean of 20 replays.
+    stage = b.graph_stage_times(gpu.stream, reps=20)
     check_outputs(gpu, d_out, out_offs, dxts, "batched timed")
     dev_bytes = b.device_bytes
     b.close()
