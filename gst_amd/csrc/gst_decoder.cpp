@@ -217,43 +217,33 @@ class PinnedRing {
   int next_ = 0;
 };
 
-// Plan blob: PlanHeader | ImageDesc[cap] | prefix[5][cap + 1].
-size_t PlanBytes(uint32_t cap) {
-  return AlignUp(sizeof(PlanHeader), 64) + AlignUp(sizeof(ImageDesc) * cap, 64) +
-         AlignUp(sizeof(uint32_t) * gst::kNumStages * (cap + 1), 64);
-}
-
-void BuildPlan(const std::vector<ImageDesc> &descs, uint32_t cap, uint32_t groups_per_unit,
-               std::vector<uint8_t> *blob, uint32_t units_out[gst::kNumStages]) {
-  blob->assign(PlanBytes(cap), 0);
-  PlanHeader *ph = reinterpret_cast<PlanHeader *>(blob->data());
-  ImageDesc *im = reinterpret_cast<ImageDesc *>(blob->data() + AlignUp(sizeof(PlanHeader), 64));
-  uint32_t *pre = reinterpret_cast<uint32_t *>(blob->data() + AlignUp(sizeof(PlanHeader), 64) +
-                                               AlignUp(sizeof(ImageDesc) * cap, 64));
+// Plan blob of n images (layout in gst_internal.h); its size is PlanBytesFor(n).
+void BuildPlan(const std::vector<ImageDesc> &descs, uint32_t groups_per_unit, std::vector<uint8_t> *blob,
+               uint32_t units_out[gst::kNumStages]) {
   const uint32_t n = static_cast<uint32_t>(descs.size());
+  blob->assign(gst::PlanBytesFor(n), 0);
+  PlanHeader *ph = reinterpret_cast<PlanHeader *>(blob->data());
+  uint32_t *pre = reinterpret_cast<uint32_t *>(blob->data() + gst::kPlanHeaderBytes);
+  ImageDesc *im = reinterpret_cast<ImageDesc *>(blob->data() + gst::PlanImagesOffset(n));
   ph->n_images = n;
-  ph->capacity = cap;
   ph->groups_per_unit = groups_per_unit;
   for (uint32_t st = 0; st < gst::kNumStages; ++st) {
     uint32_t acc = 0;
     for (uint32_t i = 0; i < n; ++i) {
-      pre[st * (cap + 1) + i] = acc;
+      pre[st * (n + 1) + i] = acc;
       acc += gst::StageUnitsOf(descs[i], st, groups_per_unit);
     }
-    for (uint32_t i = n; i <= cap; ++i) pre[st * (cap + 1) + i] = acc;
+    pre[st * (n + 1) + n] = acc;
     ph->units[st] = acc;
     units_out[st] = acc;
   }
   memcpy(im, descs.data(), sizeof(ImageDesc) * n);
 }
 
-KernelArgs PlanArgs(uint8_t *d_blob, uint32_t cap, uint32_t *err, uint32_t *tables, uint32_t *status) {
+KernelArgs PlanArgs(uint8_t *d_blob, uint32_t *err, uint32_t *tables, uint32_t *status) {
   KernelArgs a;
   memset(&a, 0, sizeof(a));
   a.plan = reinterpret_cast<const PlanHeader *>(d_blob);
-  a.images = reinterpret_cast<const ImageDesc *>(d_blob + AlignUp(sizeof(PlanHeader), 64));
-  a.prefix = reinterpret_cast<const uint32_t *>(d_blob + AlignUp(sizeof(PlanHeader), 64) +
-                                                AlignUp(sizeof(ImageDesc) * cap, 64));
   a.err = err;
   a.tables = tables;
   a.table_status = status;
@@ -308,7 +298,7 @@ CallLayout LayoutCall(const gst_header *hdrs, uint32_t n) {
   size_t off = 0;
   L.tables = off; off += AlignUp(4ull * n * gst::kTableSize * 4, kAlign);
   L.status = off; off += AlignUp(4ull * n * 4, kAlign);
-  L.plan = off; off += n > 1 ? AlignUp(PlanBytes(n), kAlign) : 0;
+  L.plan = off; off += n > 1 ? AlignUp(gst::PlanBytesFor(n), kAlign) : 0;
   L.images = off;
   L.carves.resize(n);
   L.carve_off.resize(n);
@@ -367,11 +357,11 @@ int DecodeStaging(const gst_header *hdrs, uint32_t n, hipStream_t s, const void 
   } else {
     std::vector<uint8_t> blob;
     uint32_t units[gst::kNumStages];
-    BuildPlan(descs, n, GroupsPerUnit(cfg), &blob, units);
+    BuildPlan(descs, GroupsPerUnit(cfg), &blob, units);
     uint8_t *d_blob = sc.ptr + L.plan;
     rc = PinnedRing::Get().Upload(d_blob, blob.data(), blob.size(), s);
     if (rc == GST_OK) {
-      const KernelArgs a = PlanArgs(d_blob, n, err, tables, status);
+      const KernelArgs a = PlanArgs(d_blob, err, tables, status);
       for (uint32_t st = 0; st < gst::kNumStages; ++st) grids[st] = gst::SinglePassGrid(st, units[st], cfg);
       rc = LaunchAll(a, grids, cfg, s);
     }
@@ -619,8 +609,7 @@ struct gst_batch {
 namespace {
 
 KernelArgs BatchArgs(gst_batch *b) {
-  return PlanArgs(b->dmem + b->plan_off, b->cfg.capacity_images,
-                  reinterpret_cast<uint32_t *>(b->dmem + b->err_off),
+  return PlanArgs(b->dmem + b->plan_off, reinterpret_cast<uint32_t *>(b->dmem + b->err_off),
                   reinterpret_cast<uint32_t *>(b->dmem + b->tables_off),
                   reinterpret_cast<uint32_t *>(b->dmem + b->status_off));
 }
@@ -629,7 +618,7 @@ KernelArgs BatchArgs(gst_batch *b) {
 // copies it to the device on `s` through the pinned staging buffer.
 int UploadPlan(gst_batch *b, hipStream_t s, uint32_t units[gst::kNumStages]) {
   std::vector<uint8_t> blob;
-  BuildPlan(b->bound, b->cfg.capacity_images, GroupsPerUnit(b->stage), &blob, units);
+  BuildPlan(b->bound, GroupsPerUnit(b->stage), &blob, units);
   if (b->blob_pending) GST_HIP(hipEventSynchronize(b->blob_ev));
   memcpy(b->h_blob, blob.data(), blob.size());
   GST_HIP(hipMemcpyAsync(b->dmem + b->plan_off, b->h_blob, blob.size(), hipMemcpyHostToDevice, s));
@@ -728,7 +717,7 @@ int gst_batch_create(const gst_batch_config *cfg, gst_batch **out) {
   const uint64_t cap = cfg->capacity_images, B = cfg->capacity_blocks;
   const uint64_t Pcap = cfg->capacity_palette ? cfg->capacity_palette : 4 * B;
   size_t off = 0;
-  b->blob_bytes = PlanBytes(cfg->capacity_images);
+  b->blob_bytes = gst::PlanBytesFor(cfg->capacity_images);
   b->plan_off = off; off += AlignUp(b->blob_bytes, kAlign);
   b->tables_off = off; off += AlignUp(4 * cap * gst::kTableSize * 4, kAlign);
   b->status_off = off; off += AlignUp(4 * cap * 4, kAlign);

@@ -76,20 +76,28 @@ __host__ __device__ inline uint32_t StageUnitsOf(const ImageDesc &d, uint32_t st
   }
 }
 
-// Device plan for batched decodes: header, then ImageDesc[capacity], then
-// prefix[kNumStages][capacity + 1] (work units per stage, exclusive prefix over images).
+// Device plan for batched decodes of n images: the header (padded to 64 bytes), then
+// prefix[kNumStages][n + 1] (work units per stage, exclusive prefix over the images),
+// then ImageDesc[n].  Everything after the header moves with n, so a plan is one
+// contiguous copy of exactly its used bytes, and kernels find the parts from n.
 struct PlanHeader {
   uint32_t n_images;
-  uint32_t capacity;
   uint32_t units[kNumStages];
   uint32_t groups_per_unit;
 };
+constexpr size_t kPlanHeaderBytes = 64;
+static_assert(sizeof(PlanHeader) <= kPlanHeaderBytes, "plan header");
+
+__host__ __device__ inline size_t PlanImagesOffset(uint32_t n) {
+  return kPlanHeaderBytes + ((sizeof(uint32_t) * kNumStages * (n + 1) + 63) & ~static_cast<size_t>(63));
+}
+__host__ __device__ inline size_t PlanBytesFor(uint32_t n) {
+  return PlanImagesOffset(n) + sizeof(ImageDesc) * n;
+}
 
 // Kernel arguments (by value).  plan == nullptr selects the single image in `single`.
 struct KernelArgs {
   const PlanHeader *plan;
-  const ImageDesc *images;
-  const uint32_t *prefix;
   uint32_t *err;
   uint32_t *tables;        // n_tables x 2048 packed entries
   uint32_t *table_status;  // n_tables: 1 = valid

@@ -24,7 +24,8 @@ struct Located {
 __device__ __forceinline__ Located Locate(const KernelArgs &a, uint32_t stage, uint32_t u) {
   if (!a.plan) return {0u, u};
   const uint32_t n = a.plan->n_images;
-  const uint32_t *pre = a.prefix + stage * (a.plan->capacity + 1);
+  const uint32_t *pre = reinterpret_cast<const uint32_t *>(reinterpret_cast<const uint8_t *>(a.plan) +
+                                                           kPlanHeaderBytes) + stage * (n + 1);
   uint32_t lo = 0, hi = n;
   while (hi - lo > 1) {
     const uint32_t mid = (lo + hi) >> 1;
@@ -34,7 +35,9 @@ __device__ __forceinline__ Located Locate(const KernelArgs &a, uint32_t stage, u
 }
 
 __device__ __forceinline__ ImageDesc LoadDesc(const KernelArgs &a, uint32_t img) {
-  return a.plan ? a.images[img] : a.single;
+  if (!a.plan) return a.single;
+  return reinterpret_cast<const ImageDesc *>(reinterpret_cast<const uint8_t *>(a.plan) +
+                                             PlanImagesOffset(a.plan->n_images))[img];
 }
 
 // Element k of a 4-entry descriptor array without dynamic indexing (which would put
