@@ -58,6 +58,18 @@ def stage_bytes(hdrs, fused=False):
     }
 
 
+def measured_traffic(stage, n_images, requested):
+    """HBM bytes per launch of `stage` (read + write) from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary of this workload, scaled by the copy calibration;
+    None for another corpus size or a stage the summary lacks."""
+    path = os.path.join(ROOT, "profiles", "c3_batch_traffic.json")
+    if requested != 128 or n_images != 128 or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        per = json.load(f)["bytes_per_launch"].get(stage)
+    return None if per is None else per["read"] + per["write"]
+
+
 def pipeline_bytes(hdrs):
     b_pipe = sum((h.stream_bytes + 2048) + 42 * h.num_blocks + 2 * h.palette_bytes + 65536 for h in hdrs)
     b_min = sum((h.stream_bytes + 2048) + 8 * h.num_blocks for h in hdrs)
@@ -351,7 +363,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": measured_traffic(dom, len(hdrs), args.images),
         },
         "pipeline_roofline": {
             "B_pipe_bytes": b_pipe,
