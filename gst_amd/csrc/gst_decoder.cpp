@@ -144,6 +144,7 @@ StageConfig DefaultStageConfig(bool rgb) {
   s.ans_groups_per_wave = 2;
   s.wavelet_waves = 1;
   s.fused_waves = rgb ? 0 : 4;
+  s.ans_build_tables = 1;
   s.rgb = rgb ? 1 : 0;
   return s;
 }
@@ -704,6 +705,7 @@ int gst_batch_create(const gst_batch_config *cfg, gst_batch **out) {
   b->stage.ans_groups_per_wave = cfg->ans_groups_per_wave ? cfg->ans_groups_per_wave : 2;
   b->stage.ans_waves = cfg->ans_waves_per_group ? cfg->ans_waves_per_group : 4;
   b->stage.wavelet_waves = cfg->wavelet_waves_per_group ? cfg->wavelet_waves_per_group : 1;
+  b->stage.ans_build_tables = (cfg->separate_table_stage || b->stage.ans_waves < 4) ? 0 : 1;
   b->stage.fused_waves = (cfg->rgb_output || cfg->separate_index_scan)
                              ? 0
                              : (cfg->assembly_waves_per_group ? cfg->assembly_waves_per_group : 4);
@@ -788,29 +790,41 @@ int gst_batch_autotune(gst_batch *b, gst_stream stream, uint32_t reps, gst_tune_
   int rc;
   float ms = 0.f;
 
-  // Tables: every later stage reads them; one grid per table is the natural shape.
-  if ((rc = TimeStage(b, gst::kStageTables, grids[gst::kStageTables], best, s, reps, &ms)) != GST_OK) return rc;
-  res.stage_ms[gst::kStageTables] = ms;
+  // Tables as their own launch (one work-group per table); the rANS shapes that load
+  // them are charged this time, the ones that build their table in LDS are not.
+  best.ans_build_tables = 0;
+  grids[gst::kStageTables] = 4 * b->n_bound;
+  float t_tables = 0.f;
+  if ((rc = TimeStage(b, gst::kStageTables, grids[gst::kStageTables], best, s, reps, &t_tables)) != GST_OK)
+    return rc;
 
   // rANS: streams per wavefront (32 = one group per wave, 64 = two) x waves per
-  // work-group x persistent grid; each shape needs its own plan (units of W*GPW groups).
+  // work-group x table source x persistent grid; each shape needs its own plan
+  // (units of W*GPW groups).
   static const uint32_t kAnsShapes[][2] = {{1, 2}, {1, 4}, {1, 8}, {2, 1}, {2, 2}, {2, 4}, {2, 8}};
-  float best_ms = 1e30f;
+  float best_ms = 1e30f, best_ans = 0.f;
   for (const auto &shape : kAnsShapes) {
-    StageConfig c = best;
-    c.ans_groups_per_wave = shape[0];
-    c.ans_waves = shape[1];
-    b->stage = c;
-    if ((rc = UploadPlan(b, s, units)) != GST_OK) return rc;
-    const uint32_t cand[] = {cus, 2 * cus, 4 * cus, 8 * cus, units[gst::kStageAns]};
-    for (uint32_t g : cand) {
-      if ((rc = TimeStage(b, gst::kStageAns, g, c, s, reps, &ms)) != GST_OK) return rc;
-      ++res.candidates;
-      if (ms < best_ms) {
-        best_ms = ms;
-        best.ans_groups_per_wave = c.ans_groups_per_wave;
-        best.ans_waves = c.ans_waves;
-        grids[gst::kStageAns] = g;
+    for (uint32_t build = 0; build < 2; ++build) {
+      if (build && shape[1] < 4) continue;  // the LDS build needs 256 threads
+      StageConfig c = best;
+      c.ans_groups_per_wave = shape[0];
+      c.ans_waves = shape[1];
+      c.ans_build_tables = build;
+      b->stage = c;
+      if ((rc = UploadPlan(b, s, units)) != GST_OK) return rc;
+      const uint32_t cand[] = {cus, 2 * cus, 4 * cus, 8 * cus, units[gst::kStageAns]};
+      for (uint32_t g : cand) {
+        if ((rc = TimeStage(b, gst::kStageAns, g, c, s, reps, &ms)) != GST_OK) return rc;
+        ++res.candidates;
+        const float total = ms + (build ? 0.f : t_tables);
+        if (total < best_ms) {
+          best_ms = total;
+          best_ans = ms;
+          best.ans_groups_per_wave = c.ans_groups_per_wave;
+          best.ans_waves = c.ans_waves;
+          best.ans_build_tables = build;
+          grids[gst::kStageAns] = g;
+        }
       }
     }
   }
@@ -818,7 +832,9 @@ int gst_batch_autotune(gst_batch *b, gst_stream stream, uint32_t reps, gst_tune_
   if ((rc = UploadPlan(b, s, units)) != GST_OK) return rc;
   // Leave valid symbols behind for the stages that read them.
   if ((rc = TimeStage(b, gst::kStageAns, grids[gst::kStageAns], best, s, reps, &ms)) != GST_OK) return rc;
-  res.stage_ms[gst::kStageAns] = ms;
+  res.stage_ms[gst::kStageTables] = best.ans_build_tables ? 0.f : t_tables;
+  res.stage_ms[gst::kStageAns] = best_ans;
+  if (best.ans_build_tables) grids[gst::kStageTables] = 0;
 
   // Wavelet: waves per work-group (4 tiles per wave) x grid.
   best_ms = 1e30f;
@@ -900,6 +916,7 @@ int gst_batch_autotune(gst_batch *b, gst_stream stream, uint32_t reps, gst_tune_
   res.ans_waves_per_group = best.ans_waves;
   res.wavelet_waves_per_group = best.wavelet_waves;
   res.assembly_waves_per_group = best.fused_waves;
+  res.ans_builds_tables = best.ans_build_tables;
   res.grid_ans = grids[gst::kStageAns];
   res.grid_indices = grids[gst::kStageIndices];
   res.grid_wavelet = grids[gst::kStageWavelet];
@@ -1039,6 +1056,7 @@ int StageLaunch(uint32_t stage, const gst_header *hdr, const uint8_t *d_freqs, c
   a.table_status = const_cast<uint32_t *>(d_tables) + 4 * gst::kTableSize;
   StageConfig cfg = DefaultStageConfig(rgb);
   cfg.fused_waves = 0;  // each stage is its own launch here
+  cfg.ans_build_tables = 0;
   const uint32_t grid = gst::SinglePassGrid(stage, gst::StageUnitsOf(d, stage, GroupsPerUnit(cfg)), cfg);
   GST_HIP(gst::LaunchStage(stage, a, grid, cfg, s));
   return GST_OK;

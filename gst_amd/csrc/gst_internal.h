@@ -108,6 +108,8 @@ struct KernelArgs {
 struct StageConfig {
   uint32_t ans_waves;            // waves per rANS work-group (1, 2, 4, 8)
   uint32_t ans_groups_per_wave;  // 1 or 2
+  uint32_t ans_build_tables;     // 1: the rANS kernel builds its table in LDS (4 or 8 waves);
+                                 // the table stage launches nothing
   uint32_t wavelet_waves;        // waves per wavelet work-group (1, 2, 4); a wave takes 2 units
   uint32_t fused_waves;          // 0: separate index scan and assembly; else waves per
                                  // work-group of the fused scan + DXT1 assembly (1, 2, 4)
@@ -120,6 +122,7 @@ inline uint32_t SinglePassGrid(uint32_t stage, uint32_t units, const StageConfig
     const uint32_t per = 2 * c.wavelet_waves;
     return (units + per - 1) / per;
   }
+  if (c.ans_build_tables && stage == kStageTables) return 0;  // built by the rANS kernel
   if (c.fused_waves && stage == kStageIndices) return 0;  // folded into the assembly
   if (c.fused_waves && stage == kStageAssemble) return (units + c.fused_waves - 1) / c.fused_waves;
   return units;

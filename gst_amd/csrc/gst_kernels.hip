@@ -287,11 +287,72 @@ __device__ __forceinline__ const uint8_t *Uniform(const uint8_t *p) {
   return reinterpret_cast<const uint8_t *>(lo | (hi << 32));
 }
 
-template <uint32_t WAVES, uint32_t GPW>
-__global__ __launch_bounds__(WAVES * 64) void AnsDecodeKernel(KernelArgs a) {
+// The work of BuildTableKernel for one table, straight into the rANS kernel's LDS
+// arrays (256 <= THREADS): scan of the 256 frequencies by the first four waves, then
+// every slot's symbol by the same binary lifting.  A table that does not sum to 2048
+// becomes the identity table.  Returns whether the table is valid; ends with a
+// work-group barrier.
+template <uint32_t THREADS>
+__device__ __forceinline__ bool BuildTableInLds(const uint16_t *fr, uint32_t *tab_fs, uint16_t *tab_bias,
+                                                uint32_t *cum, uint32_t *freq, uint32_t *wsum) {
+  static_assert(THREADS >= 256, "one thread per frequency");
+  const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  uint32_t f = 0, v = 0;
+  if (t < 256) {
+    f = fr[t];
+    v = f;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t n = __shfl_up(v, off);
+      if (lane >= static_cast<uint32_t>(off)) v += n;
+    }
+    if (lane == 63) wsum[wave] = v;
+  }
+  __syncthreads();
+  if (t < 256) {
+    uint32_t add = 0;
+    for (uint32_t w = 0; w < wave; ++w) add += wsum[w];
+    cum[t] = v + add - f;
+    freq[t] = f;
+  }
+  const bool valid = wsum[0] + wsum[1] + wsum[2] + wsum[3] == kTableSize;
+  __syncthreads();
+#pragma unroll 1
+  for (uint32_t slot = t; slot < kTableSize; slot += THREADS) {
+    if (valid) {
+      uint32_t x;
+      if (cum[255] <= slot) {
+        x = 255;
+      } else {
+        uint32_t pos = 0;  // number of cum[0..254] <= slot
+#pragma unroll
+        for (uint32_t s = 128; s; s >>= 1)
+          if (cum[pos + s - 1] <= slot) pos += s;
+        x = pos - 1;
+      }
+      tab_fs[slot] = freq[x] | (x << 24);
+      tab_bias[slot] = static_cast<uint16_t>(slot - cum[x]);
+    } else {
+      tab_fs[slot] = kTableSize;
+      tab_bias[slot] = static_cast<uint16_t>(slot);
+    }
+  }
+  __syncthreads();
+  return valid;
+}
+
+// kBuild: build the stream's table from its frequencies in LDS instead of loading
+// the one BuildTableKernel wrote (the decode pipeline then skips that launch).
+template <uint32_t WAVES, uint32_t GPW, bool kBuild>
+// With 4 or 8 waves LDS allows 7 waves per SIMD; hold the VGPRs to that (at most 72).
+__global__ __launch_bounds__(WAVES * 64, WAVES >= 4 ? 7 : 1)
+void AnsDecodeKernel(KernelArgs a) {
   __shared__ uint32_t tab_fs[kTableSize];
   __shared__ uint16_t tab_bias[kTableSize];
   __shared__ uint16_t ring[WAVES][2][kRingWords];
+  __shared__ uint32_t build_cum[kBuild ? 256 : 1];
+  __shared__ uint32_t build_freq[kBuild ? 256 : 1];
+  __shared__ uint32_t build_wsum[4];
   constexpr uint32_t kGroupsPerUnit = WAVES * GPW;
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint32_t half = GPW == 2 ? (lane >> 5) : 0;
@@ -308,18 +369,22 @@ __global__ __launch_bounds__(WAVES * 64) void AnsDecodeKernel(KernelArgs a) {
     }
     k = Uniform(k);
     local = Uniform(local);
-    const uint32_t tix = Uniform(d.table_base + k);
     __syncthreads();
-    {
+    bool table_ok;
+    if constexpr (kBuild) {
+      const uint16_t *fr = reinterpret_cast<const uint16_t *>(Uniform(d.freqs + 512 * k));
+      table_ok = BuildTableInLds<WAVES * 64>(fr, tab_fs, tab_bias, build_cum, build_freq, build_wsum);
+    } else {
+      const uint32_t tix = Uniform(d.table_base + k);
       const uint32_t *src = a.tables + static_cast<size_t>(tix) * kTableSize;
       for (uint32_t i = threadIdx.x; i < kTableSize; i += WAVES * 64) {
         const uint32_t e = src[i];
         tab_fs[i] = (e & 0xFFFu) | (e & 0xFF000000u);
         tab_bias[i] = static_cast<uint16_t>((e >> 12) & 0x7FFu);
       }
+      table_ok = a.table_status[tix] != 0;
+      __syncthreads();
     }
-    const bool table_ok = a.table_status[tix] != 0;
-    __syncthreads();
     if (!table_ok && threadIdx.x == 0) atomicOr(a.err, kErrFrequencies);
 
     const uint32_t G = Uniform(Pick4(d.groups, k));
@@ -785,9 +850,9 @@ __global__ __launch_bounds__(256) void AssembleRgbKernel(KernelArgs a) {
   }
 }
 
-template <uint32_t WAVES, uint32_t GPW>
+template <uint32_t WAVES, uint32_t GPW, bool kBuild = false>
 hipError_t LaunchAns(const KernelArgs &args, uint32_t grid, hipStream_t stream) {
-  hipLaunchKernelGGL((AnsDecodeKernel<WAVES, GPW>), dim3(grid), dim3(WAVES * 64), 0, stream, args);
+  hipLaunchKernelGGL((AnsDecodeKernel<WAVES, GPW, kBuild>), dim3(grid), dim3(WAVES * 64), 0, stream, args);
   return hipGetLastError();
 }
 
@@ -798,10 +863,18 @@ hipError_t LaunchStage(uint32_t stage, const KernelArgs &args, uint32_t grid,
   if (grid == 0) return hipSuccess;
   switch (stage) {
     case kStageTables:
+      if (cfg.ans_build_tables) return hipSuccess;  // the rANS kernel builds them in LDS
       hipLaunchKernelGGL(BuildTableKernel, dim3(grid), dim3(256), 0, stream, args);
       return hipGetLastError();
     case kStageAns: {
       const uint32_t w = cfg.ans_waves, g = cfg.ans_groups_per_wave;
+      if (cfg.ans_build_tables) {
+        if (g == 2 && w == 4) return LaunchAns<4, 2, true>(args, grid, stream);
+        if (g == 2 && w == 8) return LaunchAns<8, 2, true>(args, grid, stream);
+        if (g == 1 && w == 4) return LaunchAns<4, 1, true>(args, grid, stream);
+        if (g == 1 && w == 8) return LaunchAns<8, 1, true>(args, grid, stream);
+        return hipErrorInvalidValue;
+      }
       if (g == 2 && w == 1) return LaunchAns<1, 2>(args, grid, stream);
       if (g == 2 && w == 2) return LaunchAns<2, 2>(args, grid, stream);
       if (g == 2 && w == 4) return LaunchAns<4, 2>(args, grid, stream);
