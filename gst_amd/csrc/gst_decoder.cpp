@@ -139,7 +139,7 @@ void FillDesc(const gst_header &h, const uint8_t *freqs, const uint8_t *streams,
 }
 
 StageConfig DefaultStageConfig(bool rgb) {
-  StageConfig s;
+  StageConfig s{};
   s.ans_waves = 4;
   s.ans_groups_per_wave = 2;
   s.wavelet_waves = 1;
@@ -709,6 +709,7 @@ int gst_batch_create(const gst_batch_config *cfg, gst_batch **out) {
   b->stage.fused_waves = (cfg->rgb_output || cfg->separate_index_scan)
                              ? 0
                              : (cfg->assembly_waves_per_group ? cfg->assembly_waves_per_group : 4);
+  b->stage.fused_wide = cfg->assembly_wide ? 1 : 0;
   b->stage.rgb = cfg->rgb_output ? 1 : 0;
   const uint32_t cus = static_cast<uint32_t>(G().cu_count > 0 ? G().cu_count : 256);
   b->grids[gst::kStageTables] = 4 * cfg->capacity_images;
@@ -859,20 +860,24 @@ int gst_batch_autotune(gst_batch *b, gst_stream stream, uint32_t reps, gst_tune_
   // (grid each).  The fused shapes go first: they read the lane totals the rANS stage
   // left at the head of the index region, which the separate scan overwrites.
   float fused_ms = 1e30f;
-  uint32_t fused_w = 0, fused_grid = 0;
+  uint32_t fused_w = 0, fused_grid = 0, fused_wide = 0;
   if (!best.rgb) {
-    for (uint32_t w : {1u, 2u, 4u}) {
-      StageConfig c = best;
-      c.fused_waves = w;
-      const uint32_t cand[] = {2 * cus, 4 * cus, 8 * cus, 16 * cus,
-                               gst::SinglePassGrid(gst::kStageAssemble, units[gst::kStageAssemble], c)};
-      for (uint32_t g : cand) {
-        if ((rc = TimeStage(b, gst::kStageAssemble, g, c, s, reps, &ms)) != GST_OK) return rc;
-        ++res.candidates;
-        if (ms < fused_ms) {
-          fused_ms = ms;
-          fused_w = w;
-          fused_grid = g;
+    for (uint32_t wide = 0; wide < 2; ++wide) {
+      for (uint32_t w : {1u, 2u, 4u}) {
+        StageConfig c = best;
+        c.fused_waves = w;
+        c.fused_wide = wide;
+        const uint32_t cand[] = {2 * cus, 4 * cus, 8 * cus, 16 * cus,
+                                 gst::SinglePassGrid(gst::kStageAssemble, units[gst::kStageAssemble], c)};
+        for (uint32_t g : cand) {
+          if ((rc = TimeStage(b, gst::kStageAssemble, g, c, s, reps, &ms)) != GST_OK) return rc;
+          ++res.candidates;
+          if (ms < fused_ms) {
+            fused_ms = ms;
+            fused_w = w;
+            fused_wide = wide;
+            fused_grid = g;
+          }
         }
       }
     }
@@ -897,6 +902,7 @@ int gst_batch_autotune(gst_batch *b, gst_stream stream, uint32_t reps, gst_tune_
   }
   if (fused_ms < sep_ms[0] + sep_ms[1]) {
     best.fused_waves = fused_w;
+    best.fused_wide = fused_wide;
     grids[gst::kStageIndices] = 0;
     grids[gst::kStageAssemble] = fused_grid;
     res.stage_ms[gst::kStageIndices] = 0.f;
@@ -917,6 +923,7 @@ int gst_batch_autotune(gst_batch *b, gst_stream stream, uint32_t reps, gst_tune_
   res.wavelet_waves_per_group = best.wavelet_waves;
   res.assembly_waves_per_group = best.fused_waves;
   res.ans_builds_tables = best.ans_build_tables;
+  res.assembly_wide = best.fused_wide;
   res.grid_ans = grids[gst::kStageAns];
   res.grid_indices = grids[gst::kStageIndices];
   res.grid_wavelet = grids[gst::kStageWavelet];

@@ -113,6 +113,8 @@ struct StageConfig {
   uint32_t wavelet_waves;        // waves per wavelet work-group (1, 2, 4); a wave takes 2 units
   uint32_t fused_waves;          // 0: separate index scan and assembly; else waves per
                                  // work-group of the fused scan + DXT1 assembly (1, 2, 4)
+  uint32_t fused_wide;           // fused kernel layout: 0 = 4 blocks per lane x 4 segments,
+                                 // 1 = 16 consecutive blocks per lane
   uint32_t rgb;                  // 1: RGB8 assembly
 };
 

@@ -788,6 +788,75 @@ __global__ __launch_bounds__(WAVES * 64) void AssembleFusedKernel(KernelArgs a) 
   }
 }
 
+// The same unit with 16 consecutive blocks per lane: 16-byte loads of the diffs and
+// of each plane, one wave scan per unit, 128 contiguous output bytes per lane.
+__device__ __forceinline__ uint32_t ByteOf(const uint4 &v, int q) {
+  const uint32_t w = q < 4 ? v.x : q < 8 ? v.y : q < 12 ? v.z : v.w;
+  return (w >> (8 * (q & 3))) & 0xFFu;
+}
+
+template <uint32_t WAVES>
+__global__ __launch_bounds__(WAVES * 64) void AssembleFusedWideKernel(KernelArgs a) {
+  constexpr uint32_t kPerLane = kAssembleBlocksPerUnit / 64;  // 16
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t wave = Uniform(threadIdx.x >> 6);
+  const uint32_t total = StageTotal(a, kStageAssemble, 1);
+  for (uint32_t u = blockIdx.x * WAVES + wave; u < total; u += gridDim.x * WAVES) {
+    const Located loc = Locate(a, kStageAssemble, u);
+    const ImageDesc d = LoadDesc(a, loc.img);
+    const uint32_t nv = d.nv;
+    const uint32_t b0 = loc.local * kAssembleBlocksPerUnit;
+    const uint32_t blk = b0 + kPerLane * lane;
+    const uint4 dv = *reinterpret_cast<const uint4 *>(d.symbols + SymbolOffset(d, 3) + blk);
+    uint4 pv[6];
+#pragma unroll
+    for (int p = 0; p < 6; ++p)
+      pv[p] = *reinterpret_cast<const uint4 *>(d.planes + static_cast<size_t>(p) * nv + blk);
+    const uint32_t g = b0 / kGroupSyms;
+    const uint32_t seg0 = (b0 % kGroupSyms) / kSymsPerLane;
+    int32_t base = 0;
+    for (uint32_t i = lane; i < g; i += 64) base += d.group_sums[i];
+    if (lane < seg0) base += d.lane_sums[g * kLanesPerGroup + lane];
+    base = WaveSum64(base);
+    int32_t run[kPerLane];
+    int32_t acc = 0;
+#pragma unroll
+    for (int q = 0; q < static_cast<int>(kPerLane); ++q) {
+      acc += static_cast<int32_t>(ByteOf(dv, q)) - 128;
+      run[q] = acc;
+    }
+    int32_t incl = acc;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int32_t n = __shfl_up(incl, off);
+      if (lane >= static_cast<uint32_t>(off)) incl += n;
+    }
+    const int32_t before = base + incl - acc;
+    const uint32_t *pal = reinterpret_cast<const uint32_t *>(d.symbols + SymbolOffset(d, 2));
+    const uint32_t words = d.palette_bytes / 4;
+    bool bad = false;
+    uint4 *dst = reinterpret_cast<uint4 *>(d.out + 8 * static_cast<size_t>(blk));
+#pragma unroll
+    for (int i = 0; i < static_cast<int>(kPerLane) / 2; ++i) {
+      uint32_t o[4];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int q = 2 * i + h;
+        const int32_t y1 = static_cast<int8_t>(ByteOf(pv[0], q)), y2 = static_cast<int8_t>(ByteOf(pv[1], q));
+        const int32_t co1 = static_cast<int8_t>(ByteOf(pv[2], q)), cg1 = static_cast<int8_t>(ByteOf(pv[3], q));
+        const int32_t co2 = static_cast<int8_t>(ByteOf(pv[4], q)), cg2 = static_cast<int8_t>(ByteOf(pv[5], q));
+        const uint32_t pi = static_cast<uint32_t>(before + run[q]);
+        uint32_t word = 0;
+        if (pi < words) word = pal[pi]; else bad = true;
+        o[2 * h] = Rgb565(y1, co1, cg1) | (Rgb565(y2, co2, cg2) << 16);
+        o[2 * h + 1] = word;
+      }
+      dst[i] = make_uint4(o[0], o[1], o[2], o[3]);
+    }
+    if (bad) atomicOr(a.err, kErrPaletteIndex);
+  }
+}
+
 __global__ __launch_bounds__(256) void AssembleRgbKernel(KernelArgs a) {
   const uint32_t total = StageTotal(a, kStageAssemble, 1);
   for (uint32_t u = blockIdx.x; u < total; u += gridDim.x) {
@@ -902,6 +971,15 @@ hipError_t LaunchStage(uint32_t stage, const KernelArgs &args, uint32_t grid,
       if (cfg.rgb) {
         if (cfg.fused_waves) return hipErrorInvalidValue;  // the fused scan writes DXT1 only
         hipLaunchKernelGGL(AssembleRgbKernel, dim3(grid), dim3(256), 0, stream, args);
+      } else if (cfg.fused_waves && cfg.fused_wide) {
+        if (cfg.fused_waves == 1)
+          hipLaunchKernelGGL(AssembleFusedWideKernel<1>, dim3(grid), dim3(64), 0, stream, args);
+        else if (cfg.fused_waves == 2)
+          hipLaunchKernelGGL(AssembleFusedWideKernel<2>, dim3(grid), dim3(128), 0, stream, args);
+        else if (cfg.fused_waves == 4)
+          hipLaunchKernelGGL(AssembleFusedWideKernel<4>, dim3(grid), dim3(256), 0, stream, args);
+        else
+          return hipErrorInvalidValue;
       } else if (cfg.fused_waves == 1) {
         hipLaunchKernelGGL(AssembleFusedKernel<1>, dim3(grid), dim3(64), 0, stream, args);
       } else if (cfg.fused_waves == 2) {

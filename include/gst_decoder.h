@@ -183,6 +183,8 @@ typedef struct gst_batch_config {
   uint32_t separate_index_scan;     /* 1: index scan and assembly as two launches */
   uint32_t separate_table_stage;    /* 1: build_table as its own launch; 0: the rANS kernel
                                        builds each table in LDS (4 or 8 waves per group) */
+  uint32_t assembly_wide;           /* fused assembly with 16 consecutive blocks per lane
+                                       (one scan per 1024 blocks) instead of 4 x 4 */
 } gst_batch_config;
 
 int gst_batch_create(const gst_batch_config *cfg, gst_batch **out);
@@ -214,6 +216,7 @@ typedef struct gst_tune_result {
   uint32_t wavelet_waves_per_group;  /* 4 tiles per wave */
   uint32_t assembly_waves_per_group; /* 0: separate index scan and assembly */
   uint32_t ans_builds_tables;        /* 1: no build_table launch, tables built in LDS */
+  uint32_t assembly_wide;            /* fused assembly: 1 = 16 consecutive blocks per lane */
   uint32_t grid_ans, grid_indices, grid_wavelet, grid_assemble;
   uint32_t candidates;               /* shapes timed */
   float stage_ms[5];                 /* median time of the chosen shape per stage */

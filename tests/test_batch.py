@@ -46,16 +46,17 @@ def test_uneven_batch_matches_oracle(device):
         b.close()
 
 
-@pytest.mark.parametrize("gpw,waves,wwaves,awaves,separate,sep_tables", [
-    (2, 4, 1, 4, False, False), (1, 4, 2, 1, False, True), (2, 8, 4, 2, False, False),
-    (1, 8, 1, 4, True, False), (2, 1, 1, 0, True, False), (1, 2, 4, 0, True, True)])
-def test_launch_shapes_agree(device, gpw, waves, wwaves, awaves, separate, sep_tables):
+@pytest.mark.parametrize("gpw,waves,wwaves,awaves,separate,sep_tables,wide", [
+    (2, 4, 1, 4, False, False, False), (1, 4, 2, 1, False, True, True), (2, 8, 4, 2, False, False, False),
+    (1, 8, 1, 4, True, False, False), (2, 1, 1, 0, True, False, False), (1, 2, 4, 0, True, True, False),
+    (2, 4, 2, 2, False, False, True), (2, 8, 1, 4, False, True, True)])
+def test_launch_shapes_agree(device, gpw, waves, wwaves, awaves, separate, sep_tables, wide):
     files, dxts = encode.make_textures(MIXED[:4])
     blocks, pal = _capacity(files)
     b = gst_amd.Batch(len(files), blocks, pal, groups_per_wave=gpw, waves_per_group=waves,
                       grids={"ans_decode": 64, "inv_wavelet": 128, "decode_indices": 16, "assemble": 32},
                       wavelet_waves=wwaves, assembly_waves=awaves, separate_index_scan=separate,
-                      separate_table_stage=sep_tables)
+                      separate_table_stage=sep_tables, assembly_wide=wide)
     try:
         _, _, d_outs = _bind(device, b, files)
         b.decode(device.stream)
