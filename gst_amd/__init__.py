@@ -135,7 +135,6 @@ def _declare(lib):
         "gst_batch_decode_timed": ([_P, _P, ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
         "gst_batch_error_bits": ([_P, _P, ctypes.POINTER(_U32)], ctypes.c_int),
         "gst_batch_autotune": ([_P, _P, _U32, ctypes.POINTER(TuneResult)], ctypes.c_int),
-        "gst_batch_graph_stage_times": ([_P, _P, _U32, ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
         "gst_batch_device_bytes": ([_P], _SZ),
         "gst_batch_destroy": ([_P], ctypes.c_int),
     }
@@ -333,12 +332,6 @@ class Batch:
         bits = _U32()
         _check(lib().gst_batch_error_bits(self._h, stream, ctypes.byref(bits)), "gst_batch_error_bits")
         return bits.value
-
-    def graph_stage_times(self, stream, reps=20):
-        """Mean per-stage time (ms) inside replays of the captured pipeline."""
-        ms = (ctypes.c_float * 5)()
-        _check(lib().gst_batch_graph_stage_times(self._h, stream, reps, ms), "gst_batch_graph_stage_times")
-        return dict(zip(STAGES, list(ms)))
 
     def autotune(self, stream, reps=5):
         """Times every launch shape on the bound batch and keeps the fastest per stage."""

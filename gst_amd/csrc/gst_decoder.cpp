@@ -600,9 +600,6 @@ struct gst_batch {
   bool blob_pending = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  hipGraph_t graph_timed = nullptr;    // the same launches with an event around each stage
-  hipGraphExec_t exec_timed = nullptr;
-  hipEvent_t stage_ev[gst::kNumStages + 1] = {};
   uint32_t n_bound = 0;
   std::vector<ImageDesc> bound;  // descriptors of the bound images (for re-planning)
 };
@@ -631,25 +628,17 @@ int UploadPlan(gst_batch *b, hipStream_t s, uint32_t units[gst::kNumStages]) {
 void DropGraph(gst_batch *b) {
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
-  if (b->exec_timed) (void)hipGraphExecDestroy(b->exec_timed);
-  if (b->graph_timed) (void)hipGraphDestroy(b->graph_timed);
   b->exec = nullptr;
   b->graph = nullptr;
-  b->exec_timed = nullptr;
-  b->graph_timed = nullptr;
 }
 
-// Captures the five stage launches on `s` into *exec; with `ev`, an event record
-// before the first stage and after each one.
-int CaptureStages(gst_batch *b, hipStream_t s, const hipEvent_t *ev, hipGraph_t *graph,
-                  hipGraphExec_t *exec) {
+// Captures the five stage launches on `s` into *exec.
+int CaptureStages(gst_batch *b, hipStream_t s, hipGraph_t *graph, hipGraphExec_t *exec) {
   GST_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   const KernelArgs a = BatchArgs(b);
-  hipError_t e = ev ? hipEventRecord(ev[0], s) : hipSuccess;
-  for (uint32_t st = 0; st < gst::kNumStages && e == hipSuccess; ++st) {
+  hipError_t e = hipSuccess;
+  for (uint32_t st = 0; st < gst::kNumStages && e == hipSuccess; ++st)
     e = gst::LaunchStage(st, a, b->grids[st], b->stage, s);
-    if (e == hipSuccess && ev) e = hipEventRecord(ev[st + 1], s);
-  }
   hipGraph_t g = nullptr;
   const hipError_t end = hipStreamEndCapture(s, &g);
   if (e == hipSuccess) e = end;
@@ -941,30 +930,7 @@ int gst_batch_decode(gst_batch *b, gst_stream stream) {
 int gst_batch_capture_graph(gst_batch *b, gst_stream stream) {
   if (!b || b->n_bound == 0) return GST_ERR_INVALID_ARGUMENT;
   if (b->exec) return GST_OK;
-  return CaptureStages(b, static_cast<hipStream_t>(stream), nullptr, &b->graph, &b->exec);
-}
-
-int gst_batch_graph_stage_times(gst_batch *b, gst_stream stream, uint32_t reps, float ms[5]) {
-  if (!b || !ms || b->n_bound == 0 || reps == 0) return GST_ERR_INVALID_ARGUMENT;
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  for (auto &e : b->stage_ev)
-    if (!e) GST_HIP(hipEventCreate(&e));
-  if (!b->exec_timed) {
-    const int rc = CaptureStages(b, s, b->stage_ev, &b->graph_timed, &b->exec_timed);
-    if (rc != GST_OK) return rc;
-  }
-  double acc[gst::kNumStages] = {};
-  for (uint32_t r = 0; r < reps; ++r) {
-    GST_HIP(hipGraphLaunch(b->exec_timed, s));
-    GST_HIP(hipEventSynchronize(b->stage_ev[gst::kNumStages]));
-    for (uint32_t st = 0; st < gst::kNumStages; ++st) {
-      float t = 0.f;
-      GST_HIP(hipEventElapsedTime(&t, b->stage_ev[st], b->stage_ev[st + 1]));
-      acc[st] += t;
-    }
-  }
-  for (uint32_t st = 0; st < gst::kNumStages; ++st) ms[st] = static_cast<float>(acc[st] / reps);
-  return GST_OK;
+  return CaptureStages(b, static_cast<hipStream_t>(stream), &b->graph, &b->exec);
 }
 
 int gst_batch_launch_graph(gst_batch *b, gst_stream stream) {
@@ -1016,8 +982,6 @@ int gst_batch_destroy(gst_batch *b) {
   if (!b) return GST_ERR_INVALID_ARGUMENT;
   if (b->dmem) (void)hipDeviceSynchronize();
   DropGraph(b);
-  for (hipEvent_t e : b->stage_ev)
-    if (e) (void)hipEventDestroy(e);
   if (b->blob_ev) (void)hipEventDestroy(b->blob_ev);
   if (b->h_blob) (void)hipHostFree(b->h_blob);
   if (b->dmem) (void)hipFree(b->dmem);

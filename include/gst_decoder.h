@@ -201,10 +201,6 @@ int gst_batch_launch_graph(gst_batch *b, gst_stream stream);
 /* Per-stage kernel time (ms) of one decode, measured with hipEvents on `stream`:
  * build_table, ans_decode, decode_indices, inv_wavelet, assemble. Blocking. */
 int gst_batch_decode_timed(gst_batch *b, gst_stream stream, float ms[5]);
-/* Mean per-stage time (ms) inside a graph replay: replays, `reps` times, a graph of
- * the same launches with a hipEvent recorded before and after each stage (captured
- * once, like gst_batch_capture_graph's), waiting for each replay.  Blocking. */
-int gst_batch_graph_stage_times(gst_batch *b, gst_stream stream, uint32_t reps, float ms[5]);
 int gst_batch_error_bits(gst_batch *b, gst_stream stream, uint32_t *bits);
 /* Non-blocking form: enqueues on `stream` a copy of the error bits into `host_bits`
  * (pinned host memory) followed by their reset; read it once `stream` has passed. */

@@ -105,17 +105,6 @@ def test_decode_timed_reports_every_stage(device):
         ms = b.decode_timed(device.stream)
         assert set(ms) == set(gst_amd.STAGES) and all(v > 0 for v in ms.values())
         assert [device.download(o) for o in outs] == dxts
-        for o in outs:
-            o.zero_()
-        g = b.graph_stage_times(device.stream, reps=3)
-        assert set(g) == set(gst_amd.STAGES)
-        assert all(g[k] > 0 for k in ("build_table", "ans_decode", "inv_wavelet", "assemble"))
-        assert [device.download(o) for o in outs] == dxts
-        b.capture_graph(device.stream)  # the plain graph is independent of the timed copy
-        for o in outs:
-            o.zero_()
-        b.launch_graph(device.stream)
-        assert [device.download(o) for o in outs] == dxts
     finally:
         b.close()
 
