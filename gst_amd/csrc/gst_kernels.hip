@@ -126,9 +126,10 @@ __global__ __launch_bounds__(256) void BuildTableKernel(KernelArgs a) {
 // Stage 2: interleaved rANS decode (ans/ans_decode.cl:97-157).
 //
 // A work-group of WAVES wave64s decodes GPW*WAVES consecutive groups of one stream
-// with that stream's table resident in LDS as two arrays, u32 {f | symbol << 24} and
-// u16 {slot - cum}, so the state update is one v_mad_u32_u24 (12 KiB per table: with
-// the rings below a 4-wave work-group takes 20 KiB, and 7 of them fit a CU).  With
+// with that stream's table resident in LDS in the 4-byte packed form BuildTableKernel
+// writes (one random LDS read per step: random table reads are where the LDS bank
+// conflicts come from; 8 KiB per table, so a 4-wave work-group with its rings takes
+// 16 KiB).  The state update is one v_mad_u32_u24 after two bit-field extracts.  With
 // GPW = 2 a wave carries two 32-stream groups, one per half-wave; each lane holds
 // one stream's state.  The reference's local-memory renormalisation mask
 // (atomic_or / popcount / atomic_and between barriers) becomes one ballot: a lane
@@ -168,8 +169,7 @@ __device__ __forceinline__ void StoreChunk(uint16_t *ring, uint32_t c, uint32_t 
 }
 
 template <bool kSum, uint32_t GPW>
-__device__ __forceinline__ void DecodeHalfGroups(const uint32_t *tab_fs, const uint16_t *tab_bias,
-                                                 uint16_t *ring_half,
+__device__ __forceinline__ void DecodeHalfGroups(const uint32_t *tab, uint16_t *ring_half,
                                                  __amdgpu_buffer_rsrc_t rs, uint32_t lane, bool active,
                                                  uint32_t x, uint32_t pA, uint32_t pB, uint32_t loA,
                                                  uint32_t loB, uint8_t *outp, int32_t *group_sum,
@@ -209,10 +209,9 @@ __device__ __forceinline__ void DecodeHalfGroups(const uint32_t *tab_fs, const u
     for (uint32_t blk = 0; blk < 64 / kRefillSteps; ++blk) {
 #pragma unroll
       for (uint32_t s = 0; s < kRefillSteps; ++s) {
-        const uint32_t slot = x & (kTableSize - 1);
-        const uint32_t fs = tab_fs[slot];
-        // (x >> 11) * f + slot - cum; the low 24 bits of fs are f.
-        x = __umul24(x >> kTableLog, fs) + tab_bias[slot];
+        // One LDS read per step: e = f | (slot - cum) << 12 | symbol << 24.
+        const uint32_t fs = tab[x & (kTableSize - 1)];
+        x = __umul24(x >> kTableLog, fs & 0xFFFu) + __builtin_amdgcn_ubfe(fs, 12, 11);
         const uint64_t m = __builtin_amdgcn_uicmp(x, kStateL, 36 /* ULT */) & active_mask;
         const uint32_t mlo = static_cast<uint32_t>(m), mhi = static_cast<uint32_t>(m >> 32);
         const uint32_t tlo = __popc(mlo), thi = __popc(mhi);
@@ -293,8 +292,8 @@ __device__ __forceinline__ const uint8_t *Uniform(const uint8_t *p) {
 // becomes the identity table.  Returns whether the table is valid; ends with a
 // work-group barrier.
 template <uint32_t THREADS>
-__device__ __forceinline__ bool BuildTableInLds(const uint16_t *fr, uint32_t *tab_fs, uint16_t *tab_bias,
-                                                uint32_t *cum, uint32_t *freq, uint32_t *wsum) {
+__device__ __forceinline__ bool BuildTableInLds(const uint16_t *fr, uint32_t *tab, uint32_t *cum, uint32_t *freq,
+                                                uint32_t *wsum) {
   static_assert(THREADS >= 256, "one thread per frequency");
   const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
   uint32_t f = 0, v = 0;
@@ -330,11 +329,9 @@ __device__ __forceinline__ bool BuildTableInLds(const uint16_t *fr, uint32_t *ta
           if (cum[pos + s - 1] <= slot) pos += s;
         x = pos - 1;
       }
-      tab_fs[slot] = freq[x] | (x << 24);
-      tab_bias[slot] = static_cast<uint16_t>(slot - cum[x]);
+      tab[slot] = freq[x] | ((slot - cum[x]) << 12) | (x << 24);
     } else {
-      tab_fs[slot] = kTableSize;
-      tab_bias[slot] = static_cast<uint16_t>(slot);
+      tab[slot] = kTableSize | (slot << 12);
     }
   }
   __syncthreads();
@@ -347,8 +344,7 @@ template <uint32_t WAVES, uint32_t GPW, bool kBuild>
 // With 4 or 8 waves LDS allows 7 waves per SIMD; hold the VGPRs to that (at most 72).
 __global__ __launch_bounds__(WAVES * 64, WAVES >= 4 ? 7 : 1)
 void AnsDecodeKernel(KernelArgs a) {
-  __shared__ uint32_t tab_fs[kTableSize];
-  __shared__ uint16_t tab_bias[kTableSize];
+  __shared__ uint32_t tab[kTableSize];
   __shared__ uint16_t ring[WAVES][2][kRingWords];
   __shared__ uint32_t build_cum[kBuild ? 256 : 1];
   __shared__ uint32_t build_freq[kBuild ? 256 : 1];
@@ -373,15 +369,11 @@ void AnsDecodeKernel(KernelArgs a) {
     bool table_ok;
     if constexpr (kBuild) {
       const uint16_t *fr = reinterpret_cast<const uint16_t *>(Uniform(d.freqs + 512 * k));
-      table_ok = BuildTableInLds<WAVES * 64>(fr, tab_fs, tab_bias, build_cum, build_freq, build_wsum);
+      table_ok = BuildTableInLds<WAVES * 64>(fr, tab, build_cum, build_freq, build_wsum);
     } else {
       const uint32_t tix = Uniform(d.table_base + k);
       const uint32_t *src = a.tables + static_cast<size_t>(tix) * kTableSize;
-      for (uint32_t i = threadIdx.x; i < kTableSize; i += WAVES * 64) {
-        const uint32_t e = src[i];
-        tab_fs[i] = (e & 0xFFFu) | (e & 0xFF000000u);
-        tab_bias[i] = static_cast<uint16_t>((e >> 12) & 0x7FFu);
-      }
+      for (uint32_t i = threadIdx.x; i < kTableSize; i += WAVES * 64) tab[i] = src[i];
       table_ok = a.table_status[tix] != 0;
       __syncthreads();
     }
@@ -415,12 +407,11 @@ void AnsDecodeKernel(KernelArgs a) {
     uint8_t *outp = d.symbols + SymbolOffset(d, k) + static_cast<size_t>(g) * kGroupSyms + j * kSymsPerLane;
     uint16_t *ring_half = ring[wave][half];
     if (k == 3)
-      DecodeHalfGroups<true, GPW>(tab_fs, tab_bias, ring_half, rs, lane, active, x, pA, pB, loA, loB, outp,
+      DecodeHalfGroups<true, GPW>(tab, ring_half, rs, lane, active, x, pA, pB, loA, loB, outp,
                                   d.group_sums + g,
                                   d.lane_sums ? d.lane_sums + g * kLanesPerGroup : nullptr, a.err);
     else
-      DecodeHalfGroups<false, GPW>(tab_fs, tab_bias, ring_half, rs, lane, active, x, pA, pB, loA, loB, outp,
-                                   nullptr,
+      DecodeHalfGroups<false, GPW>(tab, ring_half, rs, lane, active, x, pA, pB, loA, loB, outp, nullptr,
                                    nullptr, a.err);
   }
 }
