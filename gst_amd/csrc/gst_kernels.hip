@@ -600,17 +600,40 @@ __global__ __launch_bounds__(WAVES * 64) void InvWaveletKernel(KernelArgs a) {
   const uint32_t wave = Uniform(threadIdx.x >> 6);
   const uint32_t total = StageTotal(a, kStageWavelet, 1);
   uint32_t *L = lds[wave][half];
-  for (uint32_t pair = blockIdx.x * WAVES + wave; 2 * pair < total; pair += gridDim.x * WAVES) {
-    const TileRef lo = WaveletTile(a, 2 * pair, half);
-    const bool has_hi = 2 * pair + 1 < total;
-    const TileRef hi = has_hi ? WaveletTile(a, 2 * pair + 1, half) : lo;
-    const uint4 *sa = reinterpret_cast<const uint4 *>(lo.src + r * kTileDim);
-    const uint4 a0 = sa[0], a1 = sa[1];
-    uint4 b0 = make_uint4(0, 0, 0, 0), b1 = b0;
-    if (has_hi) {
-      const uint4 *sb = reinterpret_cast<const uint4 *>(hi.src + r * kTileDim);
-      b0 = sb[0];
-      b1 = sb[1];
+  const uint32_t stride = gridDim.x * WAVES;
+  // The rows of the next pair are requested before the current pair is transformed,
+  // so a wave's loads overlap its own lifting when it takes more than one pair.
+  uint32_t pair = blockIdx.x * WAVES + wave;
+  TileRef nlo{}, nhi{};
+  bool nhas_hi = false;
+  uint4 na0{}, na1{}, nb0{}, nb1{};
+  if (2 * pair < total) {
+    nlo = WaveletTile(a, 2 * pair, half);
+    nhas_hi = 2 * pair + 1 < total;
+    nhi = nhas_hi ? WaveletTile(a, 2 * pair + 1, half) : nlo;
+    na0 = reinterpret_cast<const uint4 *>(nlo.src + r * kTileDim)[0];
+    na1 = reinterpret_cast<const uint4 *>(nlo.src + r * kTileDim)[1];
+    if (nhas_hi) {
+      nb0 = reinterpret_cast<const uint4 *>(nhi.src + r * kTileDim)[0];
+      nb1 = reinterpret_cast<const uint4 *>(nhi.src + r * kTileDim)[1];
+    }
+  }
+  for (; 2 * pair < total; pair += stride) {
+    const TileRef lo = nlo, hi = nhi;
+    const bool has_hi = nhas_hi;
+    const uint4 a0 = na0, a1 = na1;
+    const uint4 b0 = nhas_hi ? nb0 : make_uint4(0, 0, 0, 0), b1 = nhas_hi ? nb1 : make_uint4(0, 0, 0, 0);
+    const uint32_t next = pair + stride;
+    if (2 * next < total) {
+      nlo = WaveletTile(a, 2 * next, half);
+      nhas_hi = 2 * next + 1 < total;
+      nhi = nhas_hi ? WaveletTile(a, 2 * next + 1, half) : nlo;
+      na0 = reinterpret_cast<const uint4 *>(nlo.src + r * kTileDim)[0];
+      na1 = reinterpret_cast<const uint4 *>(nlo.src + r * kTileDim)[1];
+      if (nhas_hi) {
+        nb0 = reinterpret_cast<const uint4 *>(nhi.src + r * kTileDim)[0];
+        nb1 = reinterpret_cast<const uint4 *>(nhi.src + r * kTileDim)[1];
+      }
     }
     const uint32_t wa[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
     const uint32_t wb[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
