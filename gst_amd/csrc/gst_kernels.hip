@@ -365,6 +365,17 @@ void AnsDecodeKernel(KernelArgs a) {
     }
     k = Uniform(k);
     local = Uniform(local);
+    // The group's offsets are requested first, so their latency overlaps the table's.
+    const uint32_t G = Uniform(Pick4(d.groups, k));
+    const uint32_t g = local * kGroupsPerUnit + wave * GPW + half;
+    const uint32_t slen = Uniform(Pick4(d.stream_len, k));
+    const __amdgpu_buffer_rsrc_t rs = MakeRsrc(Uniform(Pick4(d.streams, k)), slen);
+    bool active = (GPW == 2 || lane < 32) && g < G;
+    uint32_t end = 0, lo = 0;
+    if (active) {
+      end = __builtin_amdgcn_raw_buffer_load_b32(rs, static_cast<int>(4 * g), 0, 0);
+      lo = g == 0 ? 4 * G : __builtin_amdgcn_raw_buffer_load_b32(rs, static_cast<int>(4 * (g - 1)), 0, 0);
+    }
     __syncthreads();
     bool table_ok;
     if constexpr (kBuild) {
@@ -379,15 +390,7 @@ void AnsDecodeKernel(KernelArgs a) {
     }
     if (!table_ok && threadIdx.x == 0) atomicOr(a.err, kErrFrequencies);
 
-    const uint32_t G = Uniform(Pick4(d.groups, k));
-    const uint32_t g = local * kGroupsPerUnit + wave * GPW + half;
-    const uint32_t slen = Uniform(Pick4(d.stream_len, k));
-    const __amdgpu_buffer_rsrc_t rs = MakeRsrc(Uniform(Pick4(d.streams, k)), slen);
-    bool active = (GPW == 2 || lane < 32) && g < G;
-    uint32_t end = 0, lo = 0;
     if (active) {
-      end = __builtin_amdgcn_raw_buffer_load_b32(rs, static_cast<int>(4 * g), 0, 0);
-      lo = g == 0 ? 4 * G : __builtin_amdgcn_raw_buffer_load_b32(rs, static_cast<int>(4 * (g - 1)), 0, 0);
       const bool ok = ((end | lo) & 3) == 0 && end <= slen && lo >= 4 * G && lo <= end &&
                       end - lo >= 4 * kLanesPerGroup;
       if (!ok) {
