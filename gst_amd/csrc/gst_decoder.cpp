@@ -218,33 +218,36 @@ class PinnedRing {
   int next_ = 0;
 };
 
-// Plan blob of n images (layout in gst_internal.h); its size is PlanBytesFor(n).
-void BuildPlan(const std::vector<ImageDesc> &descs, uint32_t groups_per_unit, std::vector<uint8_t> *blob,
-               uint32_t units_out[gst::kNumStages]) {
+// The used part of a plan of n <= cap images (layout in gst_internal.h): its size is
+// PlanUsedBytes(cap, n).
+void BuildPlan(const std::vector<ImageDesc> &descs, uint32_t cap, uint32_t groups_per_unit,
+               std::vector<uint8_t> *blob, uint32_t units_out[gst::kNumStages]) {
   const uint32_t n = static_cast<uint32_t>(descs.size());
-  blob->assign(gst::PlanBytesFor(n), 0);
+  blob->assign(gst::PlanUsedBytes(cap, n), 0);
   PlanHeader *ph = reinterpret_cast<PlanHeader *>(blob->data());
-  uint32_t *pre = reinterpret_cast<uint32_t *>(blob->data() + gst::kPlanHeaderBytes);
-  ImageDesc *im = reinterpret_cast<ImageDesc *>(blob->data() + gst::PlanImagesOffset(n));
+  ImageDesc *im = reinterpret_cast<ImageDesc *>(blob->data() + gst::PlanImagesOffset(cap));
   ph->n_images = n;
   ph->groups_per_unit = groups_per_unit;
   for (uint32_t st = 0; st < gst::kNumStages; ++st) {
+    uint32_t *pre = reinterpret_cast<uint32_t *>(blob->data() + gst::PlanPrefixOffset(cap, st));
     uint32_t acc = 0;
     for (uint32_t i = 0; i < n; ++i) {
-      pre[st * (n + 1) + i] = acc;
+      pre[i] = acc;
       acc += gst::StageUnitsOf(descs[i], st, groups_per_unit);
     }
-    pre[st * (n + 1) + n] = acc;
+    pre[n] = acc;
+    for (uint32_t i = n + 1; i <= cap; ++i) pre[i] = gst::kPrefixPad;
     ph->units[st] = acc;
     units_out[st] = acc;
   }
   memcpy(im, descs.data(), sizeof(ImageDesc) * n);
 }
 
-KernelArgs PlanArgs(uint8_t *d_blob, uint32_t *err, uint32_t *tables, uint32_t *status) {
+KernelArgs PlanArgs(uint8_t *d_blob, uint32_t cap, uint32_t *err, uint32_t *tables, uint32_t *status) {
   KernelArgs a;
   memset(&a, 0, sizeof(a));
   a.plan = reinterpret_cast<const PlanHeader *>(d_blob);
+  a.plan_cap = cap;
   a.err = err;
   a.tables = tables;
   a.table_status = status;
@@ -358,11 +361,11 @@ int DecodeStaging(const gst_header *hdrs, uint32_t n, hipStream_t s, const void 
   } else {
     std::vector<uint8_t> blob;
     uint32_t units[gst::kNumStages];
-    BuildPlan(descs, GroupsPerUnit(cfg), &blob, units);
+    BuildPlan(descs, n, GroupsPerUnit(cfg), &blob, units);
     uint8_t *d_blob = sc.ptr + L.plan;
     rc = PinnedRing::Get().Upload(d_blob, blob.data(), blob.size(), s);
     if (rc == GST_OK) {
-      const KernelArgs a = PlanArgs(d_blob, err, tables, status);
+      const KernelArgs a = PlanArgs(d_blob, n, err, tables, status);
       for (uint32_t st = 0; st < gst::kNumStages; ++st) grids[st] = gst::SinglePassGrid(st, units[st], cfg);
       rc = LaunchAll(a, grids, cfg, s);
     }
@@ -607,7 +610,8 @@ struct gst_batch {
 namespace {
 
 KernelArgs BatchArgs(gst_batch *b) {
-  return PlanArgs(b->dmem + b->plan_off, reinterpret_cast<uint32_t *>(b->dmem + b->err_off),
+  return PlanArgs(b->dmem + b->plan_off, b->cfg.capacity_images,
+                  reinterpret_cast<uint32_t *>(b->dmem + b->err_off),
                   reinterpret_cast<uint32_t *>(b->dmem + b->tables_off),
                   reinterpret_cast<uint32_t *>(b->dmem + b->status_off));
 }
@@ -616,7 +620,7 @@ KernelArgs BatchArgs(gst_batch *b) {
 // copies it to the device on `s` through the pinned staging buffer.
 int UploadPlan(gst_batch *b, hipStream_t s, uint32_t units[gst::kNumStages]) {
   std::vector<uint8_t> blob;
-  BuildPlan(b->bound, GroupsPerUnit(b->stage), &blob, units);
+  BuildPlan(b->bound, b->cfg.capacity_images, GroupsPerUnit(b->stage), &blob, units);
   if (b->blob_pending) GST_HIP(hipEventSynchronize(b->blob_ev));
   memcpy(b->h_blob, blob.data(), blob.size());
   GST_HIP(hipMemcpyAsync(b->dmem + b->plan_off, b->h_blob, blob.size(), hipMemcpyHostToDevice, s));

@@ -55,7 +55,8 @@ struct ImageDesc {
   uint8_t *out;
 };
 
-__host__ __device__ inline uint32_t SymbolOffset(const ImageDesc &d, uint32_t k) {
+template <typename Desc>
+__host__ __device__ inline uint32_t SymbolOffset(const Desc &d, uint32_t k) {
   return k == 0 ? 0u : k == 1 ? 2 * d.nv : k == 2 ? 6 * d.nv : 6 * d.nv + d.palette_bytes;
 }
 
@@ -76,10 +77,13 @@ __host__ __device__ inline uint32_t StageUnitsOf(const ImageDesc &d, uint32_t st
   }
 }
 
-// Device plan for batched decodes of n images: the header (padded to 64 bytes), then
-// prefix[kNumStages][n + 1] (work units per stage, exclusive prefix over the images),
-// then ImageDesc[n].  Everything after the header moves with n, so a plan is one
-// contiguous copy of exactly its used bytes, and kernels find the parts from n.
+// Device plan for batched decodes of up to `cap` images, n of them bound: the header
+// (padded to 64 bytes), then prefix[kNumStages][cap + 1] (work units per stage,
+// exclusive prefix over the images; entries past index n hold 0xFFFFFFFF), then
+// ImageDesc[cap], of which the first n are used.  Every offset depends on cap alone,
+// which the kernels receive as an argument, so a kernel reads the header and the
+// prefix in one round trip; the used bytes (everything up to the n-th descriptor)
+// are one contiguous copy.
 struct PlanHeader {
   uint32_t n_images;
   uint32_t units[kNumStages];
@@ -87,17 +91,23 @@ struct PlanHeader {
 };
 constexpr size_t kPlanHeaderBytes = 64;
 static_assert(sizeof(PlanHeader) <= kPlanHeaderBytes, "plan header");
+constexpr uint32_t kPrefixPad = 0xFFFFFFFFu;
 
-__host__ __device__ inline size_t PlanImagesOffset(uint32_t n) {
-  return kPlanHeaderBytes + ((sizeof(uint32_t) * kNumStages * (n + 1) + 63) & ~static_cast<size_t>(63));
+__host__ __device__ inline size_t PlanPrefixOffset(uint32_t cap, uint32_t stage) {
+  return kPlanHeaderBytes + sizeof(uint32_t) * static_cast<size_t>(stage) * (cap + 1);
 }
-__host__ __device__ inline size_t PlanBytesFor(uint32_t n) {
-  return PlanImagesOffset(n) + sizeof(ImageDesc) * n;
+__host__ __device__ inline size_t PlanImagesOffset(uint32_t cap) {
+  return kPlanHeaderBytes + ((sizeof(uint32_t) * kNumStages * (cap + 1) + 63) & ~static_cast<size_t>(63));
 }
+__host__ __device__ inline size_t PlanUsedBytes(uint32_t cap, uint32_t n) {
+  return PlanImagesOffset(cap) + sizeof(ImageDesc) * n;
+}
+__host__ __device__ inline size_t PlanBytesFor(uint32_t cap) { return PlanUsedBytes(cap, cap); }
 
 // Kernel arguments (by value).  plan == nullptr selects the single image in `single`.
 struct KernelArgs {
   const PlanHeader *plan;
+  uint32_t plan_cap;       // the plan's image capacity (fixes its layout)
   uint32_t *err;
   uint32_t *tables;        // n_tables x 2048 packed entries
   uint32_t *table_status;  // n_tables: 1 = valid

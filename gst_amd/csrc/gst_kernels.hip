@@ -10,41 +10,138 @@ namespace {
 
 constexpr int32_t kBufferRsrcWord3 = 0x00020000;  // raw buffer, gfx9 family
 
-__device__ __forceinline__ uint32_t StageTotal(const KernelArgs &a, uint32_t stage,
-                                               uint32_t groups_per_unit) {
-  return a.plan ? a.plan->units[stage] : StageUnitsOf(a.single, stage, groups_per_unit);
+// Pointers in the global address space.  Loads and stores through them compile to
+// global_* instructions, which count in vmcnt only; through generic pointers they
+// would be flat_* instructions, which also count in lgkmcnt, so every wait for an
+// LDS access would wait for them as well (a sector store would stall the next
+// rANS step until the store completes).  The host compilation pass only type-checks
+// this code and has no address spaces for class-type copies, so there the aliases
+// are plain pointers.
+#if defined(__HIP_DEVICE_COMPILE__)
+template <typename T>
+using GPtr = __attribute__((address_space(1))) T *;
+template <typename T>
+using CPtr = __attribute__((address_space(4))) const T *;
+#else
+template <typename T>
+using GPtr = T *;
+template <typename T>
+using CPtr = const T *;
+#endif
+
+template <typename T, typename U>
+__device__ __forceinline__ GPtr<T> Gbl(U *p) {
+  return (GPtr<T>)(reinterpret_cast<uintptr_t>(p));
 }
+
+// Element k of a 4-entry array without dynamic indexing (which would put the array
+// in scratch memory).
+template <typename T>
+__device__ __forceinline__ T Pick4(const T (&v)[4], uint32_t k) {
+  return k == 0 ? v[0] : k == 1 ? v[1] : k == 2 ? v[2] : v[3];
+}
+
+__device__ __forceinline__ uint32_t Popc64(uint64_t m) { return static_cast<uint32_t>(__popcll(m)); }
 
 struct Located {
   uint32_t img;
   uint32_t local;
+  bool valid;  // false: u is past the stage's last unit
 };
 
-// Largest image i with prefix[i] <= u (every image has at least one unit per stage).
-__device__ __forceinline__ Located Locate(const KernelArgs &a, uint32_t stage, uint32_t u) {
-  if (!a.plan) return {0u, u};
-  const uint32_t n = a.plan->n_images;
-  const uint32_t *pre = reinterpret_cast<const uint32_t *>(reinterpret_cast<const uint8_t *>(a.plan) +
-                                                           kPlanHeaderBytes) + stage * (n + 1);
-  uint32_t lo = 0, hi = n;
-  while (hi - lo > 1) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (pre[mid] <= u) lo = mid; else hi = mid;
+// Work unit u of `stage` as (image, unit within the image).  Called by a whole wave
+// with u uniform.  The image is the last index i with prefix[i] <= u, found by
+// counting over a ballot instead of a dependent binary search: for plans of up to 255
+// images the wave loads all prefix entries in one round trip (a 64-way narrowing
+// per round trip beyond).  Padding past n_images compares above every u, so a u past
+// the last unit lands on i = n_images, whose successor entry is padding (or absent
+// when n_images = cap): the unit is invalid.  The image count itself is not needed.
+__device__ __forceinline__ Located Locate(const KernelArgs &a, uint32_t stage, uint32_t u,
+                                          uint32_t groups_per_unit) {
+  if (!a.plan) return {0u, u, u < StageUnitsOf(a.single, stage, groups_per_unit)};
+  const uint32_t len = a.plan_cap + 1;
+  const GPtr<const uint32_t> pre =
+      Gbl<const uint32_t>(reinterpret_cast<const uint8_t *>(a.plan) + PlanPrefixOffset(a.plan_cap, stage));
+  const uint32_t lane = __lane_id();
+  uint32_t lo, start, next = kPrefixPad;
+  if (len <= 256) {
+    uint32_t v[4];
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k) v[k] = lane + 64 * k < len ? pre[lane + 64 * k] : kPrefixPad;
+    uint32_t cnt = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k) cnt += Popc64(__builtin_amdgcn_ballot_w64(v[k] <= u));
+    lo = cnt - 1;  // prefix[0] = 0 <= u
+    start = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(Pick4(v, lo >> 6)),
+                                                            static_cast<int>(lo & 63)));
+    if (lo + 1 < len)
+      next = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(Pick4(v, (lo + 1) >> 6)),
+                                                             static_cast<int>((lo + 1) & 63)));
+  } else {
+    lo = 0;
+    uint32_t m = len;  // the answer lies in [lo, lo + m)
+    while (m > 64) {
+      const uint32_t step = (m + 63) / 64;
+      const uint32_t v = lane * step < m ? pre[lo + lane * step] : kPrefixPad;
+      const uint32_t c = Popc64(__builtin_amdgcn_ballot_w64(v <= u));  // >= 1: lane 0 reads prefix[lo]
+      lo += (c - 1) * step;
+      m = min(step, m - (c - 1) * step);
+    }
+    const uint32_t v = lane < m ? pre[lo + lane] : kPrefixPad;
+    const uint32_t c = Popc64(__builtin_amdgcn_ballot_w64(v <= u));
+    start = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), static_cast<int>(c - 1)));
+    lo += c - 1;
+    if (lo + 1 < len) next = pre[lo + 1];
   }
-  return {lo, u - pre[lo]};
+  return {lo, u - start, next != kPrefixPad};
 }
 
-__device__ __forceinline__ ImageDesc LoadDesc(const KernelArgs &a, uint32_t img) {
-  if (!a.plan) return a.single;
-  return reinterpret_cast<const ImageDesc *>(reinterpret_cast<const uint8_t *>(a.plan) +
-                                             PlanImagesOffset(a.plan->n_images))[img];
-}
+// An image descriptor as the kernels use it: the same fields with global pointers.
+struct DevDesc {
+  uint32_t width, height, nv, bx;
+  uint32_t palette_bytes;
+  uint32_t groups[4];
+  uint32_t stream_len[4];
+  uint32_t table_base;
+  const uint8_t *freqs;
+  const uint8_t *streams[4];  // read through buffer resources
+  GPtr<uint8_t> symbols;
+  GPtr<int8_t> planes;
+  GPtr<int32_t> indices;
+  GPtr<int32_t> group_sums;
+  GPtr<int32_t> lane_sums;  // may be null
+  GPtr<uint8_t> out;
+};
 
-// Element k of a 4-entry descriptor array without dynamic indexing (which would put
-// the descriptor copy in scratch memory).
-template <typename T>
-__device__ __forceinline__ T Pick4(const T (&v)[4], uint32_t k) {
-  return k == 0 ? v[0] : k == 1 ? v[1] : k == 2 ? v[2] : v[3];
+// The descriptor of image img: scalar loads (the plan is read-only while kernels run).
+__device__ __forceinline__ DevDesc LoadDesc(const KernelArgs &a, uint32_t img) {
+  ImageDesc s;
+  if (a.plan) {
+    const CPtr<ImageDesc> p = (CPtr<ImageDesc>)(reinterpret_cast<uintptr_t>(a.plan) + PlanImagesOffset(a.plan_cap));
+    s = p[img];
+  } else {
+    s = a.single;
+  }
+  DevDesc d;
+  d.width = s.width;
+  d.height = s.height;
+  d.nv = s.nv;
+  d.bx = s.bx;
+  d.palette_bytes = s.palette_bytes;
+  for (int k = 0; k < 4; ++k) {
+    d.groups[k] = s.groups[k];
+    d.stream_len[k] = s.stream_len[k];
+    d.streams[k] = s.streams[k];
+  }
+  d.table_base = s.table_base;
+  d.freqs = s.freqs;
+  d.symbols = Gbl<uint8_t>(s.symbols);
+  d.planes = Gbl<int8_t>(s.planes);
+  d.indices = Gbl<int32_t>(s.indices);
+  d.group_sums = Gbl<int32_t>(s.group_sums);
+  d.lane_sums = Gbl<int32_t>(s.lane_sums);
+  d.out = Gbl<uint8_t>(s.out);
+  return d;
 }
 
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t MakeRsrc(const void *base, uint32_t bytes) {
@@ -78,9 +175,9 @@ __global__ __launch_bounds__(256) void BuildTableKernel(KernelArgs a) {
   const uint32_t total = a.plan ? a.plan->n_images * 4 : 4;
   for (uint32_t u = blockIdx.x; u < total; u += gridDim.x) {
     const uint32_t img = u >> 2, k = u & 3;
-    const ImageDesc d = LoadDesc(a, img);
+    const DevDesc d = LoadDesc(a, img);
     const uint32_t tix = d.table_base + k;
-    const uint16_t *fr = reinterpret_cast<const uint16_t *>(d.freqs + 512 * k);
+    const GPtr<const uint16_t> fr = Gbl<const uint16_t>(d.freqs + 512 * k);
     const uint32_t f = fr[t];
     uint32_t v = f;
 #pragma unroll
@@ -97,7 +194,7 @@ __global__ __launch_bounds__(256) void BuildTableKernel(KernelArgs a) {
     const uint32_t sum = wsum[0] + wsum[1] + wsum[2] + wsum[3];
     __syncthreads();
     const bool valid = sum == kTableSize;
-    uint32_t *tab = a.tables + static_cast<size_t>(tix) * kTableSize;
+    const GPtr<uint32_t> tab = Gbl<uint32_t>(a.tables + static_cast<size_t>(tix) * kTableSize);
     for (uint32_t slot = t; slot < kTableSize; slot += 256) {
       uint32_t e;
       if (valid) {
@@ -117,7 +214,7 @@ __global__ __launch_bounds__(256) void BuildTableKernel(KernelArgs a) {
       }
       tab[slot] = e;
     }
-    if (t == 0) a.table_status[tix] = valid ? 1u : 0u;
+    if (t == 0) Gbl<uint32_t>(a.table_status)[tix] = valid ? 1u : 0u;
     __syncthreads();
   }
 }
@@ -158,41 +255,75 @@ constexpr uint32_t kRefillSteps = kChunkWords / kLanesPerGroup;  // 8
 
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
-// Lane j's 16 bytes of chunk c (words 8j .. 8j+7); chunks below the stream start
-// wrap to huge offsets and read zeros.
-__device__ __forceinline__ u32x4 LoadChunk(__amdgpu_buffer_rsrc_t rs, uint32_t c, uint32_t j) {
-  return __builtin_amdgcn_raw_buffer_load_b128(rs, static_cast<int>((c * kChunkWords + 8 * j) * 2), 0, 0);
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
+// Lane l's 8 bytes of chunk c (words 4l .. 4l+3): one wave moves a whole chunk, so a
+// refill is never divergent.  Chunks below the stream start wrap to huge offsets and
+// read zeros.
+__device__ __forceinline__ u32x2 LoadChunk(__amdgpu_buffer_rsrc_t rs, uint32_t c, uint32_t l) {
+  return __builtin_amdgcn_raw_buffer_load_b64(rs, static_cast<int>((c * kChunkWords + 4 * l) * 2), 0, 0);
 }
 
-__device__ __forceinline__ void StoreChunk(uint16_t *ring, uint32_t c, uint32_t j, const u32x4 &a) {
-  *reinterpret_cast<u32x4 *>(ring + ((c * kChunkWords + 8 * j) & (kRingWords - 1))) = a;
+__device__ __forceinline__ void StoreChunk(uint16_t *ring, uint32_t c, uint32_t l, const u32x2 &a) {
+  *reinterpret_cast<u32x2 *>(ring + ((c * kChunkWords + 4 * l) & (kRingWords - 1))) = a;
+}
+
+// The first two chunks of each half's ring (top and top-1, top = the chunk holding
+// the half's word pointer - 1), requested before the table is built and written to
+// LDS after it.
+struct AnsRing {
+  u32x2 rA0, rA1, rB0, rB1;
+  uint32_t topA, topB;
+};
+
+template <uint32_t GPW>
+__device__ __forceinline__ AnsRing RequestRing(__amdgpu_buffer_rsrc_t rs, uint32_t pA, uint32_t pB,
+                                               uint32_t lane) {
+  AnsRing r;
+  r.topA = pA ? (pA - 1) / kChunkWords : 0;
+  r.topB = pB ? (pB - 1) / kChunkWords : 0;
+  r.rA0 = LoadChunk(rs, r.topA, lane);
+  r.rA1 = LoadChunk(rs, r.topA - 1, lane);
+  r.rB0 = r.rB1 = u32x2{0u, 0u};
+  if (GPW == 2) {
+    r.rB0 = LoadChunk(rs, r.topB, lane);
+    r.rB1 = LoadChunk(rs, r.topB - 1, lane);
+  }
+  return r;
 }
 
 template <bool kSum, uint32_t GPW>
-__device__ __forceinline__ void DecodeHalfGroups(const uint32_t *tab, uint16_t *ring_half,
-                                                 __amdgpu_buffer_rsrc_t rs, uint32_t lane, bool active,
-                                                 uint32_t x, uint32_t pA, uint32_t pB, uint32_t loA,
-                                                 uint32_t loB, uint8_t *outp, int32_t *group_sum,
-                                                 int32_t *lane_sum,
+__device__ __forceinline__ void DecodeHalfGroups(const uint32_t *tab, uint16_t *ring_wave, uint16_t *ring_half,
+                                                 __amdgpu_buffer_rsrc_t rs, AnsRing ring, uint32_t lane,
+                                                 bool active, uint32_t x, uint32_t pA, uint32_t pB,
+                                                 uint32_t loA, uint32_t loB, GPtr<uint8_t> outp,
+                                                 GPtr<int32_t> group_sum, GPtr<int32_t> lane_sum,
                                                  uint32_t *err) {
   const uint32_t j = lane & 31;
   uint32_t upper = lane < 32 ? 0u : 0xFFFFFFFFu;
   asm volatile("" : "+v"(upper));  // keep the half-wave mask an opaque VGPR (one v_and below)
   const uint64_t active_mask = __builtin_amdgcn_ballot_w64(active);
-  // Ring prologue: chunks top and top-1 resident, top-2 and top-3 in flight.
-  uint32_t topA = pA ? (pA - 1) / kChunkWords : 0;
-  uint32_t topB = pB ? (pB - 1) / kChunkWords : 0;
-  const uint32_t top = lane < 32 ? topA : topB;
-  {
-    const u32x4 c0 = LoadChunk(rs, top, j);
-    const u32x4 c1 = LoadChunk(rs, top - 1, j);
-    if (GPW == 2 || lane < 32) {  // one group per wave: the idle upper half shares the ring
-      StoreChunk(ring_half, top, j, c0);
-      StoreChunk(ring_half, top - 1, j, c1);
-    }
+  uint16_t *const ringA = ring_wave;
+  uint16_t *const ringB = ring_wave + kRingWords;
+  uint32_t topA = ring.topA, topB = ring.topB;
+  // Chunks top-2 and top-3 go in flight in two fixed registers by chunk parity (even
+  // chunks in pXE, odd in pXO).  A load lands in the register its chunk is later
+  // stored from, so the wait before that store covers only that load (register moves
+  // between in-flight loads made the compiler wait for every outstanding load and
+  // store, sector stores included).
+  u32x2 pAE = LoadChunk(rs, (topA & 1u) ? topA - 3 : topA - 2, lane);
+  u32x2 pAO = LoadChunk(rs, (topA & 1u) ? topA - 2 : topA - 3, lane);
+  u32x2 pBE = {0u, 0u}, pBO = {0u, 0u};
+  if (GPW == 2) {
+    pBE = LoadChunk(rs, (topB & 1u) ? topB - 3 : topB - 2, lane);
+    pBO = LoadChunk(rs, (topB & 1u) ? topB - 2 : topB - 3, lane);
   }
-  u32x4 pend0 = LoadChunk(rs, top - 2, j);
-  u32x4 pend1 = LoadChunk(rs, top - 3, j);
+  StoreChunk(ringA, topA, lane, ring.rA0);
+  StoreChunk(ringA, topA - 1, lane, ring.rA1);
+  if (GPW == 2) {
+    StoreChunk(ringB, topB, lane, ring.rB0);
+    StoreChunk(ringB, topB - 1, lane, ring.rB1);
+  }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -236,15 +367,32 @@ __device__ __forceinline__ void DecodeHalfGroups(const uint32_t *tab, uint16_t *
       // crossed at most one chunk boundary, and the next 8 steps read only chunks
       // top and top-1 (top = chunk of pointer - 1).
       const bool crossA = pA < topA * kChunkWords, crossB = pB < topB * kChunkWords;
+      // Per crossed half (uniform branches, all 64 lanes): store chunk top-2 from the
+      // register of its parity and request chunk top-4 (same parity) into it.
+      const bool oddA = (topA & 1u) != 0, oddB = (topB & 1u) != 0;
+      if (crossA && oddA) {
+        StoreChunk(ringA, topA - 2, lane, pAO);
+        asm volatile("" ::: "memory");
+        pAO = LoadChunk(rs, topA - 4, lane);
+      }
+      if (crossA && !oddA) {
+        StoreChunk(ringA, topA - 2, lane, pAE);
+        asm volatile("" ::: "memory");
+        pAE = LoadChunk(rs, topA - 4, lane);
+      }
+      if (GPW == 2 && crossB && oddB) {
+        StoreChunk(ringB, topB - 2, lane, pBO);
+        asm volatile("" ::: "memory");
+        pBO = LoadChunk(rs, topB - 4, lane);
+      }
+      if (GPW == 2 && crossB && !oddB) {
+        StoreChunk(ringB, topB - 2, lane, pBE);
+        asm volatile("" ::: "memory");
+        pBE = LoadChunk(rs, topB - 4, lane);
+      }
+      topA -= crossA ? 1u : 0u;
+      topB -= crossB ? 1u : 0u;
       if (crossA || crossB) {
-        if (lane < 32 ? crossA : crossB) {
-          const uint32_t t = lane < 32 ? topA : topB;
-          StoreChunk(ring_half, t - 2, j, pend0);
-          pend0 = pend1;
-          pend1 = LoadChunk(rs, t - 4, j);
-        }
-        topA -= crossA ? 1u : 0u;
-        topB -= crossB ? 1u : 0u;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -258,7 +406,7 @@ __device__ __forceinline__ void DecodeHalfGroups(const uint32_t *tab, uint16_t *
       sum += static_cast<int32_t>(bytes) - 64 * 128;
     }
     if (active) {
-      uint4 *dst = reinterpret_cast<uint4 *>(outp + (kSymsPerLane - 64) - 64 * sector);
+      const GPtr<uint4> dst = (GPtr<uint4>)(outp + (kSymsPerLane - 64) - 64 * sector);
 #pragma unroll
       for (uint32_t i = 0; i < 4; ++i)
         dst[i] = make_uint4(acc[4 * i], acc[4 * i + 1], acc[4 * i + 2], acc[4 * i + 3]);
@@ -286,19 +434,22 @@ __device__ __forceinline__ const uint8_t *Uniform(const uint8_t *p) {
   return reinterpret_cast<const uint8_t *>(lo | (hi << 32));
 }
 
+// Work-group barrier that orders LDS only: waits for the wave's own LDS operations,
+// not for its global loads in flight (a __syncthreads fence would wait for those too).
+__device__ __forceinline__ void LdsBarrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
 // The work of BuildTableKernel for one table, straight into the rANS kernel's LDS
-// arrays (256 <= THREADS): scan of the 256 frequencies by the first four waves, then
-// every slot's symbol by the same binary lifting.  A table that does not sum to 2048
-// becomes the identity table.  Returns whether the table is valid; ends with a
-// work-group barrier.
+// arrays (256 <= THREADS): thread t < 256 brings frequency f[t]; a scan by the first
+// four waves, then every slot's symbol by the same binary lifting.  A table that does
+// not sum to 2048 becomes the identity table.  Returns whether the table is valid;
+// ends with a work-group barrier.
 template <uint32_t THREADS>
-__device__ __forceinline__ bool BuildTableInLds(const uint16_t *fr, uint32_t *tab, uint32_t *cum, uint32_t *freq,
+__device__ __forceinline__ bool BuildTableInLds(uint32_t f, uint32_t *tab, uint32_t *cum, uint32_t *freq,
                                                 uint32_t *wsum) {
   static_assert(THREADS >= 256, "one thread per frequency");
-  const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  uint32_t f = 0, v = 0;
+  const uint32_t t = threadIdx.x, lane = t & 63, wave = Uniform(t >> 6);
+  uint32_t v = 0;
   if (t < 256) {
-    f = fr[t];
     v = f;
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
@@ -307,7 +458,7 @@ __device__ __forceinline__ bool BuildTableInLds(const uint16_t *fr, uint32_t *ta
     }
     if (lane == 63) wsum[wave] = v;
   }
-  __syncthreads();
+  LdsBarrier();
   if (t < 256) {
     uint32_t add = 0;
     for (uint32_t w = 0; w < wave; ++w) add += wsum[w];
@@ -315,7 +466,7 @@ __device__ __forceinline__ bool BuildTableInLds(const uint16_t *fr, uint32_t *ta
     freq[t] = f;
   }
   const bool valid = wsum[0] + wsum[1] + wsum[2] + wsum[3] == kTableSize;
-  __syncthreads();
+  LdsBarrier();
 #pragma unroll 1
   for (uint32_t slot = t; slot < kTableSize; slot += THREADS) {
     if (valid) {
@@ -334,12 +485,16 @@ __device__ __forceinline__ bool BuildTableInLds(const uint16_t *fr, uint32_t *ta
       tab[slot] = kTableSize | (slot << 12);
     }
   }
-  __syncthreads();
+  LdsBarrier();
   return valid;
 }
 
 // kBuild: build the stream's table from its frequencies in LDS instead of loading
 // the one BuildTableKernel wrote (the decode pipeline then skips that launch).
+// A work-unit starts with a chain of dependent loads (plan -> descriptor -> group
+// offsets -> lane states and ring chunks), all from HBM in a full pipeline; the
+// frequencies go out with the offsets and the states and chunks before the table is
+// built, so the LDS build overlaps their latency.
 template <uint32_t WAVES, uint32_t GPW, bool kBuild>
 // With 4 or 8 waves LDS allows 7 waves per SIMD; hold the VGPRs to that (at most 72).
 __global__ __launch_bounds__(WAVES * 64, WAVES >= 4 ? 7 : 1)
@@ -350,13 +505,13 @@ void AnsDecodeKernel(KernelArgs a) {
   __shared__ uint32_t build_freq[kBuild ? 256 : 1];
   __shared__ uint32_t build_wsum[4];
   constexpr uint32_t kGroupsPerUnit = WAVES * GPW;
-  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t lane = threadIdx.x & 63, wave = Uniform(threadIdx.x >> 6);
   const uint32_t half = GPW == 2 ? (lane >> 5) : 0;
   const uint32_t j = lane & 31;
-  const uint32_t total = StageTotal(a, kStageAns, kGroupsPerUnit);
-  for (uint32_t u = blockIdx.x; u < total; u += gridDim.x) {
-    const Located loc = Locate(a, kStageAns, u);
-    const ImageDesc d = LoadDesc(a, loc.img);
+  for (uint32_t u = blockIdx.x;; u += gridDim.x) {
+    const Located loc = Locate(a, kStageAns, u, kGroupsPerUnit);
+    if (!loc.valid) break;
+    const DevDesc d = LoadDesc(a, loc.img);
     uint32_t k = 0, local = loc.local;
     for (; k < 3; ++k) {
       const uint32_t uk = (Pick4(d.groups, k) + kGroupsPerUnit - 1) / kGroupsPerUnit;
@@ -365,57 +520,58 @@ void AnsDecodeKernel(KernelArgs a) {
     }
     k = Uniform(k);
     local = Uniform(local);
-    // The group's offsets are requested first, so their latency overlaps the table's.
     const uint32_t G = Uniform(Pick4(d.groups, k));
     const uint32_t g = local * kGroupsPerUnit + wave * GPW + half;
     const uint32_t slen = Uniform(Pick4(d.stream_len, k));
     const __amdgpu_buffer_rsrc_t rs = MakeRsrc(Uniform(Pick4(d.streams, k)), slen);
-    bool active = (GPW == 2 || lane < 32) && g < G;
-    uint32_t end = 0, lo = 0;
-    if (active) {
-      end = __builtin_amdgcn_raw_buffer_load_b32(rs, static_cast<int>(4 * g), 0, 0);
-      lo = g == 0 ? 4 * G : __builtin_amdgcn_raw_buffer_load_b32(rs, static_cast<int>(4 * (g - 1)), 0, 0);
-    }
-    __syncthreads();
-    bool table_ok;
+    // Straight-line loads (out-of-range offsets read zeros): a load under a branch
+    // makes the wait for it land at the join, after later loads have gone out.
+    const uint32_t end = __builtin_amdgcn_raw_buffer_load_b32(rs, static_cast<int>(4 * g), 0, 0);
+    const uint32_t lo_word = __builtin_amdgcn_raw_buffer_load_b32(rs, static_cast<int>(4 * (g - 1)), 0, 0);
+    uint32_t f = 0;
     if constexpr (kBuild) {
-      const uint16_t *fr = reinterpret_cast<const uint16_t *>(Uniform(d.freqs + 512 * k));
-      table_ok = BuildTableInLds<WAVES * 64>(fr, tab, build_cum, build_freq, build_wsum);
-    } else {
-      const uint32_t tix = Uniform(d.table_base + k);
-      const uint32_t *src = a.tables + static_cast<size_t>(tix) * kTableSize;
-      for (uint32_t i = threadIdx.x; i < kTableSize; i += WAVES * 64) tab[i] = src[i];
-      table_ok = a.table_status[tix] != 0;
-      __syncthreads();
+      const GPtr<const uint16_t> fr = Gbl<const uint16_t>(Uniform(d.freqs + 512 * k));
+      f = fr[threadIdx.x & 255];
     }
-    if (!table_ok && threadIdx.x == 0) atomicOr(a.err, kErrFrequencies);
-
-    if (active) {
-      const bool ok = ((end | lo) & 3) == 0 && end <= slen && lo >= 4 * G && lo <= end &&
-                      end - lo >= 4 * kLanesPerGroup;
-      if (!ok) {
-        if (j == 0) atomicOr(a.err, kErrStreamOffsets);
-        active = false;
-      }
-    }
-    const uint32_t x = active
-        ? __builtin_amdgcn_raw_buffer_load_b32(rs, static_cast<int>(end - 4 * kLanesPerGroup + 4 * j), 0, 0)
-        : 0u;
+    const uint32_t lo = g == 0 ? 4 * G : lo_word;
+    const bool ok = ((end | lo) & 3) == 0 && end <= slen && lo >= 4 * G && lo <= end &&
+                    end - lo >= 4 * kLanesPerGroup;
+    const bool in_stream = (GPW == 2 || lane < 32) && g < G;
+    if (in_stream && !ok && j == 0) atomicOr(a.err, kErrStreamOffsets);
+    const bool active = in_stream && ok;
+    const uint32_t x_word =
+        __builtin_amdgcn_raw_buffer_load_b32(rs, static_cast<int>(end - 4 * kLanesPerGroup + 4 * j), 0, 0);
+    const uint32_t x = active ? x_word : 0u;
     const uint32_t p = active ? (end - 4 * kLanesPerGroup) >> 1 : 0u;
     const uint32_t plo = active ? lo >> 1 : 0u;
     const uint32_t pA = __builtin_amdgcn_readlane(p, 0);
     const uint32_t loA = __builtin_amdgcn_readlane(plo, 0);
     const uint32_t pB = GPW == 2 ? __builtin_amdgcn_readlane(p, 32) : 0u;
     const uint32_t loB = GPW == 2 ? __builtin_amdgcn_readlane(plo, 32) : 0u;
-    uint8_t *outp = d.symbols + SymbolOffset(d, k) + static_cast<size_t>(g) * kGroupSyms + j * kSymsPerLane;
+    const AnsRing chunks = RequestRing<GPW>(rs, pA, pB, lane);
+
+    LdsBarrier();  // every wave is done with the previous unit's table and ring
+    bool table_ok;
+    if constexpr (kBuild) {
+      table_ok = BuildTableInLds<WAVES * 64>(f, tab, build_cum, build_freq, build_wsum);
+    } else {
+      const uint32_t tix = Uniform(d.table_base + k);
+      const GPtr<const uint32_t> src = Gbl<const uint32_t>(a.tables + static_cast<size_t>(tix) * kTableSize);
+      for (uint32_t i = threadIdx.x; i < kTableSize; i += WAVES * 64) tab[i] = src[i];
+      table_ok = Gbl<const uint32_t>(a.table_status)[tix] != 0;
+      LdsBarrier();
+    }
+    if (!table_ok && threadIdx.x == 0) atomicOr(a.err, kErrFrequencies);
+
+    const GPtr<uint8_t> outp = d.symbols + SymbolOffset(d, k) + static_cast<size_t>(g) * kGroupSyms + j * kSymsPerLane;
     uint16_t *ring_half = ring[wave][half];
     if (k == 3)
-      DecodeHalfGroups<true, GPW>(tab, ring_half, rs, lane, active, x, pA, pB, loA, loB, outp,
-                                  d.group_sums + g,
+      DecodeHalfGroups<true, GPW>(tab, ring[wave][0], ring_half, rs, chunks, lane, active, x, pA, pB, loA, loB,
+                                  outp, d.group_sums + g,
                                   d.lane_sums ? d.lane_sums + g * kLanesPerGroup : nullptr, a.err);
     else
-      DecodeHalfGroups<false, GPW>(tab, ring_half, rs, lane, active, x, pA, pB, loA, loB, outp, nullptr,
-                                   nullptr, a.err);
+      DecodeHalfGroups<false, GPW>(tab, ring[wave][0], ring_half, rs, chunks, lane, active, x, pA, pB, loA, loB,
+                                   outp, nullptr, nullptr, a.err);
   }
 }
 
@@ -431,10 +587,10 @@ __global__ __launch_bounds__(256) void DecodeIndicesKernel(KernelArgs a) {
   __shared__ uint32_t wtot[4];
   __shared__ uint32_t gprefix;
   const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const uint32_t total = StageTotal(a, kStageIndices, 1);
-  for (uint32_t u = blockIdx.x; u < total; u += gridDim.x) {
-    const Located loc = Locate(a, kStageIndices, u);
-    const ImageDesc d = LoadDesc(a, loc.img);
+  for (uint32_t u = blockIdx.x;; u += gridDim.x) {
+    const Located loc = Locate(a, kStageIndices, u, 1);
+    if (!loc.valid) break;
+    const DevDesc d = LoadDesc(a, loc.img);
     const uint32_t g = loc.local;
     if (wave == 0) {
       uint32_t s = 0;
@@ -443,9 +599,10 @@ __global__ __launch_bounds__(256) void DecodeIndicesKernel(KernelArgs a) {
       for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
       if (lane == 0) gprefix = s;
     }
-    const uint8_t *diffs = d.symbols + SymbolOffset(d, 3) + static_cast<size_t>(g) * kGroupSyms + 32 * t;
-    const uint4 q0 = reinterpret_cast<const uint4 *>(diffs)[0];
-    const uint4 q1 = reinterpret_cast<const uint4 *>(diffs)[1];
+    const GPtr<const uint4> diffs =
+        (GPtr<const uint4>)(d.symbols + SymbolOffset(d, 3) + static_cast<size_t>(g) * kGroupSyms + 32 * t);
+    const uint4 q0 = diffs[0];
+    const uint4 q1 = diffs[1];
     const uint32_t words[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
     uint32_t run[32];
     uint32_t acc = 0;
@@ -464,7 +621,7 @@ __global__ __launch_bounds__(256) void DecodeIndicesKernel(KernelArgs a) {
     __syncthreads();
     uint32_t offset = gprefix + incl - acc;
     for (uint32_t w = 0; w < wave; ++w) offset += wtot[w];
-    uint4 *dst = reinterpret_cast<uint4 *>(d.indices + static_cast<size_t>(g) * kGroupSyms + 32 * t);
+    const GPtr<uint4> dst = (GPtr<uint4>)(d.indices + static_cast<size_t>(g) * kGroupSyms + 32 * t);
 #pragma unroll
     for (int i = 0; i < 8; ++i)
       dst[i] = make_uint4(run[4 * i] + offset, run[4 * i + 1] + offset, run[4 * i + 2] + offset,
@@ -573,12 +730,12 @@ __device__ __forceinline__ void WaveletLevel(s16x2 (&v)[kTileDim], uint32_t *L, 
 }
 
 struct TileRef {
-  const uint8_t *src;  // tile-linear symbols (1024 bytes)
-  int8_t *dst;         // top-left of the tile in the raster plane
-  uint32_t pitch;      // plane row length (blocks)
+  GPtr<const uint8_t> src;  // tile-linear symbols (1024 bytes)
+  GPtr<int8_t> dst;         // top-left of the tile in the raster plane
+  uint32_t pitch;           // plane row length (blocks)
 };
 
-__device__ __forceinline__ TileRef TileAt(const ImageDesc &d, uint32_t tile) {
+__device__ __forceinline__ TileRef TileAt(const DevDesc &d, uint32_t tile) {
   const uint32_t per_plane = d.nv / kTileSyms, tiles_x = d.bx / kTileDim;
   const uint32_t plane = tile / per_plane, t = tile - plane * per_plane;
   const uint32_t ty = t / tiles_x, tx = t - ty * tiles_x;
@@ -587,13 +744,16 @@ __device__ __forceinline__ TileRef TileAt(const ImageDesc &d, uint32_t tile) {
           d.planes + base + static_cast<size_t>(ty) * kTileDim * d.bx + tx * kTileDim, d.bx};
 }
 
-// Tile `half` of wavelet unit `u` (uniform address math, one select per lane).
-__device__ __forceinline__ TileRef WaveletTile(const KernelArgs &a, uint32_t u, uint32_t half) {
-  const Located loc = Locate(a, kStageWavelet, u);
-  const ImageDesc d = LoadDesc(a, loc.img);
+// Tile `half` of wavelet unit `u`, if the unit exists (uniform address math, one
+// select per lane; whole-wave call).
+__device__ __forceinline__ bool WaveletTile(const KernelArgs &a, uint32_t u, uint32_t half, TileRef *t) {
+  const Located loc = Locate(a, kStageWavelet, u, 1);
+  if (!loc.valid) return false;
+  const DevDesc d = LoadDesc(a, loc.img);
   const TileRef t0 = TileAt(d, loc.local * kWaveletTilesPerUnit);
   const TileRef t1 = TileAt(d, loc.local * kWaveletTilesPerUnit + 1);
-  return half ? t1 : t0;
+  *t = half ? t1 : t0;
+  return true;
 }
 
 template <uint32_t WAVES>
@@ -601,41 +761,37 @@ __global__ __launch_bounds__(WAVES * 64) void InvWaveletKernel(KernelArgs a) {
   __shared__ uint32_t lds[WAVES][2][kTileDim * kLdsStride];
   const uint32_t lane = threadIdx.x & 63, half = lane >> 5, r = lane & 31;
   const uint32_t wave = Uniform(threadIdx.x >> 6);
-  const uint32_t total = StageTotal(a, kStageWavelet, 1);
   uint32_t *L = lds[wave][half];
   const uint32_t stride = gridDim.x * WAVES;
   // The rows of the next pair are requested before the current pair is transformed,
   // so a wave's loads overlap its own lifting when it takes more than one pair.
   uint32_t pair = blockIdx.x * WAVES + wave;
   TileRef nlo{}, nhi{};
-  bool nhas_hi = false;
   uint4 na0{}, na1{}, nb0{}, nb1{};
-  if (2 * pair < total) {
-    nlo = WaveletTile(a, 2 * pair, half);
-    nhas_hi = 2 * pair + 1 < total;
-    nhi = nhas_hi ? WaveletTile(a, 2 * pair + 1, half) : nlo;
-    na0 = reinterpret_cast<const uint4 *>(nlo.src + r * kTileDim)[0];
-    na1 = reinterpret_cast<const uint4 *>(nlo.src + r * kTileDim)[1];
+  bool nhas_lo = WaveletTile(a, 2 * pair, half, &nlo);
+  bool nhas_hi = nhas_lo && WaveletTile(a, 2 * pair + 1, half, &nhi);
+  if (nhas_lo) {
+    na0 = ((GPtr<const uint4>)(nlo.src + r * kTileDim))[0];
+    na1 = ((GPtr<const uint4>)(nlo.src + r * kTileDim))[1];
     if (nhas_hi) {
-      nb0 = reinterpret_cast<const uint4 *>(nhi.src + r * kTileDim)[0];
-      nb1 = reinterpret_cast<const uint4 *>(nhi.src + r * kTileDim)[1];
+      nb0 = ((GPtr<const uint4>)(nhi.src + r * kTileDim))[0];
+      nb1 = ((GPtr<const uint4>)(nhi.src + r * kTileDim))[1];
     }
   }
-  for (; 2 * pair < total; pair += stride) {
+  for (; nhas_lo; pair += stride) {
     const TileRef lo = nlo, hi = nhi;
     const bool has_hi = nhas_hi;
     const uint4 a0 = na0, a1 = na1;
     const uint4 b0 = nhas_hi ? nb0 : make_uint4(0, 0, 0, 0), b1 = nhas_hi ? nb1 : make_uint4(0, 0, 0, 0);
     const uint32_t next = pair + stride;
-    if (2 * next < total) {
-      nlo = WaveletTile(a, 2 * next, half);
-      nhas_hi = 2 * next + 1 < total;
-      nhi = nhas_hi ? WaveletTile(a, 2 * next + 1, half) : nlo;
-      na0 = reinterpret_cast<const uint4 *>(nlo.src + r * kTileDim)[0];
-      na1 = reinterpret_cast<const uint4 *>(nlo.src + r * kTileDim)[1];
+    nhas_lo = WaveletTile(a, 2 * next, half, &nlo);
+    nhas_hi = nhas_lo && WaveletTile(a, 2 * next + 1, half, &nhi);
+    if (nhas_lo) {
+      na0 = ((GPtr<const uint4>)(nlo.src + r * kTileDim))[0];
+      na1 = ((GPtr<const uint4>)(nlo.src + r * kTileDim))[1];
       if (nhas_hi) {
-        nb0 = reinterpret_cast<const uint4 *>(nhi.src + r * kTileDim)[0];
-        nb1 = reinterpret_cast<const uint4 *>(nhi.src + r * kTileDim)[1];
+        nb0 = ((GPtr<const uint4>)(nhi.src + r * kTileDim))[0];
+        nb1 = ((GPtr<const uint4>)(nhi.src + r * kTileDim))[1];
       }
     }
     const uint32_t wa[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
@@ -663,11 +819,11 @@ __global__ __launch_bounds__(WAVES * 64) void InvWaveletKernel(KernelArgs a) {
       oa[w] = __builtin_amdgcn_perm(t23, t01, 0x05040100u);
       ob[w] = __builtin_amdgcn_perm(t23, t01, 0x07060302u);
     }
-    uint4 *da = reinterpret_cast<uint4 *>(lo.dst + static_cast<size_t>(r) * lo.pitch);
+    const GPtr<uint4> da = (GPtr<uint4>)(lo.dst + static_cast<size_t>(r) * lo.pitch);
     da[0] = make_uint4(oa[0], oa[1], oa[2], oa[3]);
     da[1] = make_uint4(oa[4], oa[5], oa[6], oa[7]);
     if (has_hi) {
-      uint4 *db = reinterpret_cast<uint4 *>(hi.dst + static_cast<size_t>(r) * hi.pitch);
+      const GPtr<uint4> db = (GPtr<uint4>)(hi.dst + static_cast<size_t>(r) * hi.pitch);
       db[0] = make_uint4(ob[0], ob[1], ob[2], ob[3]);
       db[1] = make_uint4(ob[4], ob[5], ob[6], ob[7]);
     }
@@ -696,18 +852,18 @@ __device__ __forceinline__ int SByte(uint32_t w, int q) {
 }
 
 __global__ __launch_bounds__(256) void AssembleDxtKernel(KernelArgs a) {
-  const uint32_t total = StageTotal(a, kStageAssemble, 1);
-  for (uint32_t u = blockIdx.x; u < total; u += gridDim.x) {
-    const Located loc = Locate(a, kStageAssemble, u);
-    const ImageDesc d = LoadDesc(a, loc.img);
+  for (uint32_t u = blockIdx.x;; u += gridDim.x) {
+    const Located loc = Locate(a, kStageAssemble, u, 1);
+    if (!loc.valid) break;
+    const DevDesc d = LoadDesc(a, loc.img);
     const uint32_t nv = d.nv;
     const size_t b0 = static_cast<size_t>(loc.local) * kAssembleBlocksPerUnit + 4 * threadIdx.x;
     uint32_t pw[6];
 #pragma unroll
     for (int p = 0; p < 6; ++p)
-      pw[p] = *reinterpret_cast<const uint32_t *>(d.planes + static_cast<size_t>(p) * nv + b0);
-    const int4 ix = *reinterpret_cast<const int4 *>(d.indices + b0);
-    const uint32_t *pal = reinterpret_cast<const uint32_t *>(d.symbols + SymbolOffset(d, 2));
+      pw[p] = *(GPtr<const uint32_t>)(d.planes + static_cast<size_t>(p) * nv + b0);
+    const int4 ix = *(GPtr<const int4>)(d.indices + b0);
+    const GPtr<const uint32_t> pal = (GPtr<const uint32_t>)(d.symbols + SymbolOffset(d, 2));
     const uint32_t words = d.palette_bytes / 4;
     const int idx[4] = {ix.x, ix.y, ix.z, ix.w};
     uint32_t o[8];
@@ -722,7 +878,7 @@ __global__ __launch_bounds__(256) void AssembleDxtKernel(KernelArgs a) {
       o[2 * q] = ep1 | (ep2 << 16);
       o[2 * q + 1] = word;
     }
-    uint4 *dst = reinterpret_cast<uint4 *>(d.out + 8 * b0);
+    const GPtr<uint4> dst = (GPtr<uint4>)(d.out + 8 * b0);
     dst[0] = make_uint4(o[0], o[1], o[2], o[3]);
     dst[1] = make_uint4(o[4], o[5], o[6], o[7]);
     if (bad) atomicOr(a.err, kErrPaletteIndex);
@@ -747,10 +903,10 @@ __global__ __launch_bounds__(WAVES * 64) void AssembleFusedKernel(KernelArgs a) 
   constexpr uint32_t kSegments = kAssembleBlocksPerUnit / kSymsPerLane;  // 4
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t wave = Uniform(threadIdx.x >> 6);
-  const uint32_t total = StageTotal(a, kStageAssemble, 1);
-  for (uint32_t u = blockIdx.x * WAVES + wave; u < total; u += gridDim.x * WAVES) {
-    const Located loc = Locate(a, kStageAssemble, u);
-    const ImageDesc d = LoadDesc(a, loc.img);
+  for (uint32_t u = blockIdx.x * WAVES + wave;; u += gridDim.x * WAVES) {
+    const Located loc = Locate(a, kStageAssemble, u, 1);
+    if (!loc.valid) break;
+    const DevDesc d = LoadDesc(a, loc.img);
     const uint32_t nv = d.nv;
     const uint32_t b0 = loc.local * kAssembleBlocksPerUnit;
     const uint32_t g = b0 / kGroupSyms;
@@ -759,18 +915,18 @@ __global__ __launch_bounds__(WAVES * 64) void AssembleFusedKernel(KernelArgs a) 
     for (uint32_t i = lane; i < g; i += 64) base += d.group_sums[i];
     if (lane < seg0) base += d.lane_sums[g * kLanesPerGroup + lane];
     base = WaveSum64(base);
-    const uint8_t *diffs = d.symbols + SymbolOffset(d, 3);
-    const uint32_t *pal = reinterpret_cast<const uint32_t *>(d.symbols + SymbolOffset(d, 2));
+    const GPtr<const uint8_t> diffs = d.symbols + SymbolOffset(d, 3);
+    const GPtr<const uint32_t> pal = (GPtr<const uint32_t>)(d.symbols + SymbolOffset(d, 2));
     const uint32_t words = d.palette_bytes / 4;
     bool bad = false;
 #pragma unroll
     for (uint32_t seg = 0; seg < kSegments; ++seg) {
       const uint32_t blk = b0 + seg * kSymsPerLane + 4 * lane;
-      const uint32_t dw = *reinterpret_cast<const uint32_t *>(diffs + blk);
+      const uint32_t dw = *(GPtr<const uint32_t>)(diffs + blk);
       uint32_t pw[6];
 #pragma unroll
       for (int p = 0; p < 6; ++p)
-        pw[p] = *reinterpret_cast<const uint32_t *>(d.planes + static_cast<size_t>(p) * nv + blk);
+        pw[p] = *(GPtr<const uint32_t>)(d.planes + static_cast<size_t>(p) * nv + blk);
       int32_t run[4];
       int32_t acc = 0;
 #pragma unroll
@@ -797,7 +953,7 @@ __global__ __launch_bounds__(WAVES * 64) void AssembleFusedKernel(KernelArgs a) 
         o[2 * q] = ep1 | (ep2 << 16);
         o[2 * q + 1] = word;
       }
-      uint4 *dst = reinterpret_cast<uint4 *>(d.out + 8 * static_cast<size_t>(blk));
+      const GPtr<uint4> dst = (GPtr<uint4>)(d.out + 8 * static_cast<size_t>(blk));
       dst[0] = make_uint4(o[0], o[1], o[2], o[3]);
       dst[1] = make_uint4(o[4], o[5], o[6], o[7]);
     }
@@ -817,18 +973,18 @@ __global__ __launch_bounds__(WAVES * 64) void AssembleFusedWideKernel(KernelArgs
   constexpr uint32_t kPerLane = kAssembleBlocksPerUnit / 64;  // 16
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t wave = Uniform(threadIdx.x >> 6);
-  const uint32_t total = StageTotal(a, kStageAssemble, 1);
-  for (uint32_t u = blockIdx.x * WAVES + wave; u < total; u += gridDim.x * WAVES) {
-    const Located loc = Locate(a, kStageAssemble, u);
-    const ImageDesc d = LoadDesc(a, loc.img);
+  for (uint32_t u = blockIdx.x * WAVES + wave;; u += gridDim.x * WAVES) {
+    const Located loc = Locate(a, kStageAssemble, u, 1);
+    if (!loc.valid) break;
+    const DevDesc d = LoadDesc(a, loc.img);
     const uint32_t nv = d.nv;
     const uint32_t b0 = loc.local * kAssembleBlocksPerUnit;
     const uint32_t blk = b0 + kPerLane * lane;
-    const uint4 dv = *reinterpret_cast<const uint4 *>(d.symbols + SymbolOffset(d, 3) + blk);
+    const uint4 dv = *(GPtr<const uint4>)(d.symbols + SymbolOffset(d, 3) + blk);
     uint4 pv[6];
 #pragma unroll
     for (int p = 0; p < 6; ++p)
-      pv[p] = *reinterpret_cast<const uint4 *>(d.planes + static_cast<size_t>(p) * nv + blk);
+      pv[p] = *(GPtr<const uint4>)(d.planes + static_cast<size_t>(p) * nv + blk);
     const uint32_t g = b0 / kGroupSyms;
     const uint32_t seg0 = (b0 % kGroupSyms) / kSymsPerLane;
     int32_t base = 0;
@@ -849,10 +1005,10 @@ __global__ __launch_bounds__(WAVES * 64) void AssembleFusedWideKernel(KernelArgs
       if (lane >= static_cast<uint32_t>(off)) incl += n;
     }
     const int32_t before = base + incl - acc;
-    const uint32_t *pal = reinterpret_cast<const uint32_t *>(d.symbols + SymbolOffset(d, 2));
+    const GPtr<const uint32_t> pal = (GPtr<const uint32_t>)(d.symbols + SymbolOffset(d, 2));
     const uint32_t words = d.palette_bytes / 4;
     bool bad = false;
-    uint4 *dst = reinterpret_cast<uint4 *>(d.out + 8 * static_cast<size_t>(blk));
+    const GPtr<uint4> dst = (GPtr<uint4>)(d.out + 8 * static_cast<size_t>(blk));
 #pragma unroll
     for (int i = 0; i < static_cast<int>(kPerLane) / 2; ++i) {
       uint32_t o[4];
@@ -875,18 +1031,18 @@ __global__ __launch_bounds__(WAVES * 64) void AssembleFusedWideKernel(KernelArgs
 }
 
 __global__ __launch_bounds__(256) void AssembleRgbKernel(KernelArgs a) {
-  const uint32_t total = StageTotal(a, kStageAssemble, 1);
-  for (uint32_t u = blockIdx.x; u < total; u += gridDim.x) {
-    const Located loc = Locate(a, kStageAssemble, u);
-    const ImageDesc d = LoadDesc(a, loc.img);
+  for (uint32_t u = blockIdx.x;; u += gridDim.x) {
+    const Located loc = Locate(a, kStageAssemble, u, 1);
+    if (!loc.valid) break;
+    const DevDesc d = LoadDesc(a, loc.img);
     const uint32_t nv = d.nv;
     const size_t b0 = static_cast<size_t>(loc.local) * kAssembleBlocksPerUnit + 4 * threadIdx.x;
     uint32_t pw[6];
 #pragma unroll
     for (int p = 0; p < 6; ++p)
-      pw[p] = *reinterpret_cast<const uint32_t *>(d.planes + static_cast<size_t>(p) * nv + b0);
-    const int4 ix = *reinterpret_cast<const int4 *>(d.indices + b0);
-    const uint32_t *pal = reinterpret_cast<const uint32_t *>(d.symbols + SymbolOffset(d, 2));
+      pw[p] = *(GPtr<const uint32_t>)(d.planes + static_cast<size_t>(p) * nv + b0);
+    const int4 ix = *(GPtr<const int4>)(d.indices + b0);
+    const GPtr<const uint32_t> pal = (GPtr<const uint32_t>)(d.symbols + SymbolOffset(d, 2));
     const uint32_t words = d.palette_bytes / 4;
     const int idx[4] = {ix.x, ix.y, ix.z, ix.w};
     bool bad = false;
@@ -924,8 +1080,8 @@ __global__ __launch_bounds__(256) void AssembleRgbKernel(KernelArgs a) {
           bytes[3 * c + 1] = static_cast<uint32_t>(pal8[sel][1]) & 0xFFu;
           bytes[3 * c + 2] = static_cast<uint32_t>(pal8[sel][2]) & 0xFFu;
         }
-        uint32_t *dst = reinterpret_cast<uint32_t *>(
-            d.out + 3 * (static_cast<size_t>(d.width) * (4 * byi + row) + 4 * bxi));
+        const GPtr<uint32_t> dst =
+            (GPtr<uint32_t>)(d.out + 3 * (static_cast<size_t>(d.width) * (4 * byi + row) + 4 * bxi));
 #pragma unroll
         for (int w = 0; w < 3; ++w)
           dst[w] = bytes[4 * w] | (bytes[4 * w + 1] << 8) | (bytes[4 * w + 2] << 16) |
