@@ -333,6 +333,16 @@ __device__ __forceinline__ void DecodeHalfGroups(const uint32_t *tab, uint16_t *
   // back-to-back 16-byte stores (smaller pieces of a lane's segment, 256 B apart
   // across lanes, are evicted half-written and written back several times).
   for (uint32_t sector = 0; sector < kSymsPerLane / 64; ++sector) {
+    // A SIMD's waves compete for issue slots and the oldest win, so young waves
+    // would be left with most of their steps for the end, when too few waves remain
+    // to keep the SIMD busy.  Priority falls as a wave gets further ahead, which
+    // keeps the waves of a SIMD within about a sector of each other.
+    switch (sector) {
+      case 0: __builtin_amdgcn_s_setprio(3); break;
+      case 1: __builtin_amdgcn_s_setprio(2); break;
+      case 2: __builtin_amdgcn_s_setprio(1); break;
+      default: __builtin_amdgcn_s_setprio(0); break;
+    }
     uint32_t acc[16];
 #pragma unroll
     for (uint32_t i = 0; i < 16; ++i) acc[i] = 0;
