@@ -477,22 +477,28 @@ __device__ __forceinline__ bool BuildTableInLds(uint32_t f, uint32_t *tab, uint3
   }
   const bool valid = wsum[0] + wsum[1] + wsum[2] + wsum[3] == kTableSize;
   LdsBarrier();
+  // The thread's slots t + i * THREADS search four at a time, level by level, so
+  // their LDS reads overlap (four at once is what the registers left free by the
+  // in-flight ring chunks allow without spilling).
+  constexpr uint32_t kPer = kTableSize / THREADS;
+  constexpr uint32_t kAtOnce = kPer < 4 ? kPer : 4;
+  const uint32_t last = cum[255];
 #pragma unroll 1
-  for (uint32_t slot = t; slot < kTableSize; slot += THREADS) {
-    if (valid) {
-      uint32_t x;
-      if (cum[255] <= slot) {
-        x = 255;
-      } else {
-        uint32_t pos = 0;  // number of cum[0..254] <= slot
+  for (uint32_t i0 = 0; i0 < kPer; i0 += kAtOnce) {
+    uint32_t pos[kAtOnce];  // number of cum[0..254] <= slot
 #pragma unroll
-        for (uint32_t s = 128; s; s >>= 1)
-          if (cum[pos + s - 1] <= slot) pos += s;
-        x = pos - 1;
-      }
-      tab[slot] = freq[x] | ((slot - cum[x]) << 12) | (x << 24);
-    } else {
-      tab[slot] = kTableSize | (slot << 12);
+    for (uint32_t i = 0; i < kAtOnce; ++i) pos[i] = 0;
+#pragma unroll
+    for (uint32_t s = 128; s; s >>= 1) {
+#pragma unroll
+      for (uint32_t i = 0; i < kAtOnce; ++i)
+        if (cum[pos[i] + s - 1] <= t + (i0 + i) * THREADS) pos[i] += s;
+    }
+#pragma unroll
+    for (uint32_t i = 0; i < kAtOnce; ++i) {
+      const uint32_t slot = t + (i0 + i) * THREADS;
+      const uint32_t x = last <= slot ? 255u : pos[i] - 1;
+      tab[slot] = valid ? freq[x] | ((slot - cum[x]) << 12) | (x << 24) : kTableSize | (slot << 12);
     }
   }
   LdsBarrier();
