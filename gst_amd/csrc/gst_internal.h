@@ -51,9 +51,17 @@ struct ImageDesc {
   int8_t *planes;
   int32_t *indices;
   int32_t *group_sums;   // one per index-stream group
-  int32_t *lane_sums;    // one per index-stream lane (nv / 256), or null
+  int32_t *lane_sums;    // index-stream partial sums (layout at LaneSuffixBase), or null
   uint8_t *out;
 };
+
+// Partial sums of (diff - 128) over the index stream that the rANS stage leaves in
+// lane_sums, so a palette index can start at any 32-block boundary: int32 entries
+// [0, nv/256) hold, per lane (256 symbols), the inclusive prefix of lane totals over
+// its group's lanes; entries [nv/256, nv/256 + nv/32) hold, per lane, R[k] for
+// k = 0..7, the sum over the lane's positions >= 32k.
+constexpr uint32_t kRowBlocks = 32;
+__host__ __device__ inline uint32_t LaneSuffixBase(uint32_t nv) { return nv / kSymsPerLane; }
 
 template <typename Desc>
 __host__ __device__ inline uint32_t SymbolOffset(const Desc &d, uint32_t k) {

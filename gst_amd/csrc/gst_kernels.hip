@@ -298,7 +298,7 @@ __device__ __forceinline__ void DecodeHalfGroups(const uint32_t *tab, uint16_t *
                                                  bool active, uint32_t x, uint32_t pA, uint32_t pB,
                                                  uint32_t loA, uint32_t loB, GPtr<uint8_t> outp,
                                                  GPtr<int32_t> group_sum, GPtr<int32_t> lane_sum,
-                                                 uint32_t *err) {
+                                                 GPtr<int32_t> lane_suffix, uint32_t *err) {
   const uint32_t j = lane & 31;
   uint32_t upper = lane < 32 ? 0u : 0xFFFFFFFFu;
   asm volatile("" : "+v"(upper));  // keep the half-wave mask an opaque VGPR (one v_and below)
@@ -409,11 +409,20 @@ __device__ __forceinline__ void DecodeHalfGroups(const uint32_t *tab, uint16_t *
       }
     }
     if (kSum) {
-      // Byte sum of the sector (v_sad_u8 against zero), less the 64 biases.
-      uint32_t bytes = 0;
+      // Byte sums of the sector's two 32-symbol halves (v_sad_u8 against zero), less
+      // the biases.  The sector covers lane positions 192 - 64 sector + [0, 64), so
+      // its halves are sub-segments k = 6 - 2 sector and 7 - 2 sector, and the running
+      // total from the top gives the suffix sums R[k] and R[k + 1].
+      uint32_t lo = 0, hi = 0;
 #pragma unroll
-      for (uint32_t i = 0; i < 16; ++i) bytes = __builtin_amdgcn_sad_u8(acc[i], 0u, bytes);
-      sum += static_cast<int32_t>(bytes) - 64 * 128;
+      for (uint32_t i = 0; i < 8; ++i) lo = __builtin_amdgcn_sad_u8(acc[i], 0u, lo);
+#pragma unroll
+      for (uint32_t i = 8; i < 16; ++i) hi = __builtin_amdgcn_sad_u8(acc[i], 0u, hi);
+      const int32_t r_hi = sum + static_cast<int32_t>(hi) - 32 * 128;
+      const int32_t r_lo = r_hi + static_cast<int32_t>(lo) - 32 * 128;
+      sum = r_lo;
+      if (active && lane_suffix)
+        *(GPtr<int2>)(lane_suffix + 8 * j + 6 - 2 * sector) = make_int2(r_lo, r_hi);
     }
     if (active) {
       const GPtr<uint4> dst = (GPtr<uint4>)(outp + (kSymsPerLane - 64) - 64 * sector);
@@ -425,9 +434,16 @@ __device__ __forceinline__ void DecodeHalfGroups(const uint32_t *tab, uint16_t *
   const bool under = lane < 32 ? (pA < loA) : (pB < loB);
   if (active && under && j == 0) atomicOr(err, kErrStreamOffsets);
   if (kSum) {
-    if (active && lane_sum) lane_sum[j] = sum;
-    sum = WaveSumWidth32(sum);
-    if (active && j == 0) *group_sum = sum;
+    // Inclusive prefix of the lane totals over the half-wave's 32 lanes.
+    int32_t incl = sum;
+#pragma unroll
+    for (uint32_t off = 1; off < 32; off <<= 1) {
+      const int32_t n = __shfl_up(incl, off, 32);
+      if (j >= off) incl += n;
+    }
+    if (active && lane_sum) lane_sum[j] = incl;
+    const int32_t total = __shfl(incl, 31, 32);
+    if (active && j == 0) *group_sum = total;
   }
 }
 
@@ -584,10 +600,13 @@ void AnsDecodeKernel(KernelArgs a) {
     if (k == 3)
       DecodeHalfGroups<true, GPW>(tab, ring[wave][0], ring_half, rs, chunks, lane, active, x, pA, pB, loA, loB,
                                   outp, d.group_sums + g,
-                                  d.lane_sums ? d.lane_sums + g * kLanesPerGroup : nullptr, a.err);
+                                  d.lane_sums ? d.lane_sums + g * kLanesPerGroup : nullptr,
+                                  d.lane_sums ? d.lane_sums + LaneSuffixBase(d.nv) + g * kLanesPerGroup * 8
+                                              : nullptr,
+                                  a.err);
     else
       DecodeHalfGroups<false, GPW>(tab, ring[wave][0], ring_half, rs, chunks, lane, active, x, pA, pB, loA, loB,
-                                   outp, nullptr, nullptr, a.err);
+                                   outp, nullptr, nullptr, nullptr, a.err);
   }
 }
 
@@ -903,15 +922,23 @@ __global__ __launch_bounds__(256) void AssembleDxtKernel(KernelArgs a) {
 
 // Index scan fused into DXT1 assembly: one wave per 1024-block unit, which covers
 // four 256-block segments of the index stream (one rANS lane each).  The value
-// before the unit is the sum of the earlier groups' totals and the earlier lanes'
-// totals of its group, both written by the rANS stage; a segment is a 4-per-lane
-// register prefix plus a wave scan.  The palette indices never go to memory, so a
-// block costs 1 diff byte instead of 1 + 4 + 4 bytes of index traffic.  Results are
-// those of DecodeIndicesKernel followed by AssembleDxtKernel.
+// before the unit is the sum of the earlier groups' totals plus the sum before it
+// within its group, both from the partial sums the rANS stage wrote; a segment is a
+// 4-per-lane register prefix plus a wave scan.  The palette indices never go to
+// memory, so a block costs 1 diff byte instead of 1 + 4 + 4 bytes of index traffic.
+// Results are those of DecodeIndicesKernel followed by AssembleDxtKernel.
 __device__ __forceinline__ int32_t WaveSum64(int32_t v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
   return v;
+}
+
+// Sum of (diff - 128) over the blocks of b's rANS group that come before block b
+// (b a multiple of 32): the lane's inclusive prefix less the lane's suffix from b.
+__device__ __forceinline__ int32_t GroupLocalPrefix(const DevDesc &d, uint32_t b) {
+  const uint32_t lane_ix = b / kSymsPerLane;  // group * 32 + lane
+  const uint32_t k = (b % kSymsPerLane) / kRowBlocks;
+  return d.lane_sums[lane_ix] - d.lane_sums[LaneSuffixBase(d.nv) + lane_ix * 8 + k];
 }
 
 template <uint32_t WAVES>
@@ -926,11 +953,9 @@ __global__ __launch_bounds__(WAVES * 64) void AssembleFusedKernel(KernelArgs a) 
     const uint32_t nv = d.nv;
     const uint32_t b0 = loc.local * kAssembleBlocksPerUnit;
     const uint32_t g = b0 / kGroupSyms;
-    const uint32_t seg0 = (b0 % kGroupSyms) / kSymsPerLane;
     int32_t base = 0;
     for (uint32_t i = lane; i < g; i += 64) base += d.group_sums[i];
-    if (lane < seg0) base += d.lane_sums[g * kLanesPerGroup + lane];
-    base = WaveSum64(base);
+    base = WaveSum64(base) + GroupLocalPrefix(d, b0);
     const GPtr<const uint8_t> diffs = d.symbols + SymbolOffset(d, 3);
     const GPtr<const uint32_t> pal = (GPtr<const uint32_t>)(d.symbols + SymbolOffset(d, 2));
     const uint32_t words = d.palette_bytes / 4;
@@ -1002,11 +1027,9 @@ __global__ __launch_bounds__(WAVES * 64) void AssembleFusedWideKernel(KernelArgs
     for (int p = 0; p < 6; ++p)
       pv[p] = *(GPtr<const uint4>)(d.planes + static_cast<size_t>(p) * nv + blk);
     const uint32_t g = b0 / kGroupSyms;
-    const uint32_t seg0 = (b0 % kGroupSyms) / kSymsPerLane;
     int32_t base = 0;
     for (uint32_t i = lane; i < g; i += 64) base += d.group_sums[i];
-    if (lane < seg0) base += d.lane_sums[g * kLanesPerGroup + lane];
-    base = WaveSum64(base);
+    base = WaveSum64(base) + GroupLocalPrefix(d, b0);
     int32_t run[kPerLane];
     int32_t acc = 0;
 #pragma unroll
