@@ -132,8 +132,9 @@ void FillDesc(const gst_header &h, const uint8_t *freqs, const uint8_t *streams,
   d->planes = reinterpret_cast<int8_t *>(scratch + c.planes);
   d->indices = reinterpret_cast<int32_t *>(scratch + c.indices);
   d->group_sums = reinterpret_cast<int32_t *>(scratch + c.group_sums);
-  // The rANS stage leaves per-lane sums at the head of the index region; the
-  // separate index scan overwrites them afterwards, the fused assembly reads them.
+  // The rANS stage leaves the index stream's partial sums at the head of the index
+  // region; the separate index scan overwrites them afterwards, the fused assemblies
+  // read them.
   d->lane_sums = d->indices;
   d->out = out;
 }
@@ -144,6 +145,7 @@ StageConfig DefaultStageConfig(bool rgb) {
   s.ans_groups_per_wave = 2;
   s.wavelet_waves = 1;
   s.fused_waves = rgb ? 0 : 4;
+  s.tile_fused = rgb ? 0 : 1;
   s.ans_build_tables = 1;
   s.rgb = rgb ? 1 : 0;
   return s;
@@ -703,6 +705,7 @@ int gst_batch_create(const gst_batch_config *cfg, gst_batch **out) {
                              ? 0
                              : (cfg->assembly_waves_per_group ? cfg->assembly_waves_per_group : 4);
   b->stage.fused_wide = cfg->assembly_wide ? 1 : 0;
+  b->stage.tile_fused = (cfg->rgb_output || cfg->separate_index_scan || cfg->separate_wavelet) ? 0 : 1;
   b->stage.rgb = cfg->rgb_output ? 1 : 0;
   const uint32_t cus = static_cast<uint32_t>(G().cu_count > 0 ? G().cu_count : 256);
   b->grids[gst::kStageTables] = 4 * cfg->capacity_images;
@@ -830,6 +833,27 @@ int gst_batch_autotune(gst_batch *b, gst_stream stream, uint32_t reps, gst_tune_
   res.stage_ms[gst::kStageAns] = best_ans;
   if (best.ans_build_tables) grids[gst::kStageTables] = 0;
 
+  // Wavelet inside the DXT1 assembly (grid), timed before anything overwrites the
+  // rANS stage's partial sums; the separate shapes follow.
+  const bool try_tile_fused = !best.rgb && !b->cfg.separate_index_scan && !b->cfg.separate_wavelet;
+  float tile_ms = 1e30f;
+  uint32_t tile_grid = 0;
+  if (try_tile_fused) {
+    StageConfig c = best;
+    c.tile_fused = 1;
+    const uint32_t cand[] = {2 * cus, 4 * cus, 6 * cus, 8 * cus,
+                             gst::SinglePassGrid(gst::kStageAssemble, units[gst::kStageAssemble], c)};
+    for (uint32_t g : cand) {
+      if ((rc = TimeStage(b, gst::kStageAssemble, g, c, s, reps, &ms)) != GST_OK) return rc;
+      ++res.candidates;
+      if (ms < tile_ms) {
+        tile_ms = ms;
+        tile_grid = g;
+      }
+    }
+  }
+  best.tile_fused = 0;
+
   // Wavelet: waves per work-group (4 tiles per wave) x grid.
   best_ms = 1e30f;
   for (uint32_t w : {1u, 2u, 4u}) {
@@ -907,6 +931,17 @@ int gst_batch_autotune(gst_batch *b, gst_stream stream, uint32_t reps, gst_tune_
     res.stage_ms[gst::kStageIndices] = sep_ms[0];
     res.stage_ms[gst::kStageAssemble] = sep_ms[1];
   }
+  const float separate_ms = res.stage_ms[gst::kStageWavelet] + res.stage_ms[gst::kStageIndices] +
+                            res.stage_ms[gst::kStageAssemble];
+  if (try_tile_fused && tile_ms < separate_ms) {
+    best.tile_fused = 1;
+    grids[gst::kStageWavelet] = 0;
+    grids[gst::kStageIndices] = 0;
+    grids[gst::kStageAssemble] = tile_grid;
+    res.stage_ms[gst::kStageWavelet] = 0.f;
+    res.stage_ms[gst::kStageIndices] = 0.f;
+    res.stage_ms[gst::kStageAssemble] = tile_ms;
+  }
 
   b->stage = best;
   memcpy(b->grids, grids, sizeof(grids));
@@ -917,6 +952,7 @@ int gst_batch_autotune(gst_batch *b, gst_stream stream, uint32_t reps, gst_tune_
   res.assembly_waves_per_group = best.fused_waves;
   res.ans_builds_tables = best.ans_build_tables;
   res.assembly_wide = best.fused_wide;
+  res.wavelet_fused = best.tile_fused;
   res.grid_ans = grids[gst::kStageAns];
   res.grid_indices = grids[gst::kStageIndices];
   res.grid_wavelet = grids[gst::kStageWavelet];
@@ -1031,6 +1067,7 @@ int StageLaunch(uint32_t stage, const gst_header *hdr, const uint8_t *d_freqs, c
   a.table_status = const_cast<uint32_t *>(d_tables) + 4 * gst::kTableSize;
   StageConfig cfg = DefaultStageConfig(rgb);
   cfg.fused_waves = 0;  // each stage is its own launch here
+  cfg.tile_fused = 0;
   cfg.ans_build_tables = 0;
   const uint32_t grid = gst::SinglePassGrid(stage, gst::StageUnitsOf(d, stage, GroupsPerUnit(cfg)), cfg);
   GST_HIP(gst::LaunchStage(stage, a, grid, cfg, s));

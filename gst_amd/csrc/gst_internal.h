@@ -133,11 +133,16 @@ struct StageConfig {
                                  // work-group of the fused scan + DXT1 assembly (1, 2, 4)
   uint32_t fused_wide;           // fused kernel layout: 0 = 4 blocks per lane x 4 segments,
                                  // 1 = 16 consecutive blocks per lane
+  uint32_t tile_fused;           // 1: the wavelet and the index scan run inside the DXT1
+                                 // assembly (two tile positions per 3-wave work-group);
+                                 // the wavelet and index stages launch nothing
   uint32_t rgb;                  // 1: RGB8 assembly
 };
 
 // Work-groups that give every unit of `units` its own pass (no grid-stride repeat).
 inline uint32_t SinglePassGrid(uint32_t stage, uint32_t units, const StageConfig &c) {
+  if (c.tile_fused && (stage == kStageWavelet || stage == kStageIndices)) return 0;
+  if (c.tile_fused && stage == kStageAssemble) return (units + 1) / 2;  // two units a group
   if (stage == kStageWavelet) {
     const uint32_t per = 2 * c.wavelet_waves;
     return (units + per - 1) / per;

@@ -791,6 +791,38 @@ __device__ __forceinline__ bool WaveletTile(const KernelArgs &a, uint32_t u, uin
   return true;
 }
 
+// Row r of two tiles (a: 32 symbols of the first, b: of the second) through the five
+// levels, with this half-wave's transpose tile L; returns the rows' int8 results,
+// 32 bytes each.
+__device__ __forceinline__ void TransformRowPair(const uint4 &a0, const uint4 &a1, const uint4 &b0,
+                                                 const uint4 &b1, uint32_t *L, uint32_t r, uint32_t (&oa)[8],
+                                                 uint32_t (&ob)[8]) {
+  const uint32_t wa[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+  const uint32_t wb[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
+  s16x2 v[kTileDim];
+#pragma unroll
+  for (int i = 0; i < static_cast<int>(kTileDim); ++i) {
+    // Byte i&3 of the first tile's word to bits 0-7, of the second's to bits
+    // 16-23 (selector 0x0c is a zero byte), then remove the +128 bias.
+    const uint32_t q = static_cast<uint32_t>(i & 3);
+    const uint32_t sel = 0x0C000C00u | q | ((4u + q) << 16);
+    v[i] = AsPair(__builtin_amdgcn_perm(wb[i >> 2], wa[i >> 2], sel)) - static_cast<s16x2>(128);
+  }
+  WaveletLevel<2>(v, L, static_cast<int>(r));
+  WaveletLevel<4>(v, L, static_cast<int>(r));
+  WaveletLevel<8>(v, L, static_cast<int>(r));
+  WaveletLevel<16>(v, L, static_cast<int>(r));
+  WaveletLevel<32>(v, L, static_cast<int>(r));
+#pragma unroll
+  for (int w = 0; w < 8; ++w) {
+    // Low bytes of four pairs: t01 = (A0, A1, B0, B1), t23 = (A2, A3, B2, B3).
+    const uint32_t t01 = __builtin_amdgcn_perm(Bits(v[4 * w + 1]), Bits(v[4 * w]), 0x06020400u);
+    const uint32_t t23 = __builtin_amdgcn_perm(Bits(v[4 * w + 3]), Bits(v[4 * w + 2]), 0x06020400u);
+    oa[w] = __builtin_amdgcn_perm(t23, t01, 0x05040100u);
+    ob[w] = __builtin_amdgcn_perm(t23, t01, 0x07060302u);
+  }
+}
+
 template <uint32_t WAVES>
 __global__ __launch_bounds__(WAVES * 64) void InvWaveletKernel(KernelArgs a) {
   __shared__ uint32_t lds[WAVES][2][kTileDim * kLdsStride];
@@ -829,31 +861,8 @@ __global__ __launch_bounds__(WAVES * 64) void InvWaveletKernel(KernelArgs a) {
         nb1 = ((GPtr<const uint4>)(nhi.src + r * kTileDim))[1];
       }
     }
-    const uint32_t wa[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
-    const uint32_t wb[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
-    s16x2 v[kTileDim];
-#pragma unroll
-    for (int i = 0; i < static_cast<int>(kTileDim); ++i) {
-      // Byte i&3 of the first tile's word to bits 0-7, of the second's to bits
-      // 16-23 (selector 0x0c is a zero byte), then remove the +128 bias.
-      const uint32_t q = static_cast<uint32_t>(i & 3);
-      const uint32_t sel = 0x0C000C00u | q | ((4u + q) << 16);
-      v[i] = AsPair(__builtin_amdgcn_perm(wb[i >> 2], wa[i >> 2], sel)) - static_cast<s16x2>(128);
-    }
-    WaveletLevel<2>(v, L, static_cast<int>(r));
-    WaveletLevel<4>(v, L, static_cast<int>(r));
-    WaveletLevel<8>(v, L, static_cast<int>(r));
-    WaveletLevel<16>(v, L, static_cast<int>(r));
-    WaveletLevel<32>(v, L, static_cast<int>(r));
     uint32_t oa[8], ob[8];
-#pragma unroll
-    for (int w = 0; w < 8; ++w) {
-      // Low bytes of four pairs: t01 = (A0, A1, B0, B1), t23 = (A2, A3, B2, B3).
-      const uint32_t t01 = __builtin_amdgcn_perm(Bits(v[4 * w + 1]), Bits(v[4 * w]), 0x06020400u);
-      const uint32_t t23 = __builtin_amdgcn_perm(Bits(v[4 * w + 3]), Bits(v[4 * w + 2]), 0x06020400u);
-      oa[w] = __builtin_amdgcn_perm(t23, t01, 0x05040100u);
-      ob[w] = __builtin_amdgcn_perm(t23, t01, 0x07060302u);
-    }
+    TransformRowPair(a0, a1, b0, b1, L, r, oa, ob);
     const GPtr<uint4> da = (GPtr<uint4>)(lo.dst + static_cast<size_t>(r) * lo.pitch);
     da[0] = make_uint4(oa[0], oa[1], oa[2], oa[3]);
     da[1] = make_uint4(oa[4], oa[5], oa[6], oa[7]);
@@ -1069,6 +1078,162 @@ __global__ __launch_bounds__(WAVES * 64) void AssembleFusedWideKernel(KernelArgs
   }
 }
 
+// Stages 4 and 5 in one kernel: inverse wavelet, index scan and DXT1 assembly of two
+// tile positions per work-group, so the int8 planes never leave LDS (the planes
+// buffer's write and read, 12 bytes a block, are the traffic this saves).
+//
+// A work unit is two consecutive assembly units of one image, i.e. tile positions
+// t0 and t0 + 1 of the per-plane raster tile order (every plane has a multiple of 8
+// tiles, so pairs never straddle images).  Wave w transforms planes 2w and 2w + 1,
+// one per half-wave, with t0 in the low and t0 + 1 in the high 16 bits, exactly as
+// InvWaveletKernel does, and leaves the int8 rows of its four tiles in its own
+// transpose space (1 KiB a tile).  After a work-group barrier each half-wave
+// assembles tile rows: lane x of row y of position P is block (32 ty + y, 32 tx + x)
+// of that position; its six plane values are LDS bytes, its palette index is the
+// row's base plus a 32-lane scan of the diffs, and its 8 bytes join the row's 256
+// contiguous output bytes.  A row's base is the earlier groups' totals (one 64-lane
+// scan of the group sums of the groups the unit's rows touch, when at most 64; else a
+// per-row reduction) plus the sum before the row within its group.
+constexpr uint32_t kTileFusedWaves = 3;
+
+// At most 96 VGPRs: 5 waves per SIMD, about what the 25 KiB of LDS allows.
+__global__ __launch_bounds__(kTileFusedWaves * 64, 5) void WaveletAssembleKernel(KernelArgs a) {
+  __shared__ uint32_t lds[kTileFusedWaves][2][kTileDim * kLdsStride];
+  constexpr uint32_t kWaveLdsBytes = sizeof(lds[0]);
+  const uint32_t lane = threadIdx.x & 63, half = lane >> 5, r = lane & 31;
+  const uint32_t wave = Uniform(threadIdx.x >> 6);
+  uint32_t *const L = lds[wave][half];
+  for (uint32_t u = blockIdx.x;; u += gridDim.x) {
+    const Located loc = Locate(a, kStageAssemble, 2 * u, 1);
+    if (!loc.valid) break;
+    const DevDesc d = LoadDesc(a, loc.img);
+    const uint32_t nv = d.nv, bx = d.bx, tiles_x = bx / kTileDim;
+    const uint32_t t0 = loc.local, t1 = t0 + 1;
+    // Rows of plane 2 wave + half, tiles t0 and t1 (adjacent in the tile-linear symbols).
+    const GPtr<const uint8_t> src = d.symbols + static_cast<size_t>(2 * wave + half) * nv +
+                                    static_cast<size_t>(t0) * kTileSyms + r * kTileDim;
+    const uint4 a0 = ((GPtr<const uint4>)src)[0], a1 = ((GPtr<const uint4>)src)[1];
+    const uint4 b0 = ((GPtr<const uint4>)(src + kTileSyms))[0], b1 = ((GPtr<const uint4>)(src + kTileSyms))[1];
+
+    // Totals of the groups before each of the (at most 64) groups the rows touch:
+    // lane i holds the sum over groups < g_min + i.
+    const uint32_t ty0 = t0 / tiles_x, tx0 = t0 - ty0 * tiles_x;
+    const uint32_t ty1 = t1 / tiles_x, tx1 = t1 - ty1 * tiles_x;
+    const uint32_t g_min = (ty0 * kTileDim * bx + tx0 * kTileDim) / kGroupSyms;
+    const uint32_t g_max = ((ty1 * kTileDim + kTileDim - 1) * bx + tx1 * kTileDim) / kGroupSyms;
+    const bool few_groups = g_max - g_min < 64;
+    int32_t gpre = 0;
+    if (few_groups) {
+      int32_t before = 0;
+      for (uint32_t i = lane; i < g_min; i += 64) before += d.group_sums[i];
+      before = WaveSum64(before);
+      const int32_t gs = lane <= g_max - g_min ? d.group_sums[g_min + lane] : 0;
+      int32_t incl = gs;
+#pragma unroll
+      for (uint32_t off = 1; off < 64; off <<= 1) {
+        const int32_t n = __shfl_up(incl, off);
+        if (lane >= off) incl += n;
+      }
+      gpre = before + incl - gs;
+    }
+
+    uint32_t oa[8], ob[8];
+    TransformRowPair(a0, a1, b0, b1, L, r, oa, ob);
+    // Tile (2 half + P) of this wave's space: row r at byte 32 r.
+    uint4 *const da = reinterpret_cast<uint4 *>(&lds[wave][0][0] + (2 * half + 0) * (kTileSyms / 4) + r * 8);
+    uint4 *const db = reinterpret_cast<uint4 *>(&lds[wave][0][0] + (2 * half + 1) * (kTileSyms / 4) + r * 8);
+    da[0] = make_uint4(oa[0], oa[1], oa[2], oa[3]);
+    da[1] = make_uint4(oa[4], oa[5], oa[6], oa[7]);
+    db[0] = make_uint4(ob[0], ob[1], ob[2], ob[3]);
+    db[1] = make_uint4(ob[4], ob[5], ob[6], ob[7]);
+    LdsBarrier();
+
+    const GPtr<const uint8_t> diffs = d.symbols + SymbolOffset(d, 3);
+    const GPtr<const uint32_t> pal = (GPtr<const uint32_t>)(d.symbols + SymbolOffset(d, 2));
+    const uint32_t words = d.palette_bytes / 4;
+    const uint8_t *const tiles = reinterpret_cast<const uint8_t *>(&lds[0][0][0]);
+    bool bad = false;
+    // 64 tile rows (2 positions x 32) in row groups of 8: lane l takes blocks
+    // 4 (l % 8) .. +3 of row l / 8 of its group, and wave w groups w, w + 3, w + 6
+    // (< 8).  Each phase runs over all of a wave's groups, so their loads are in
+    // flight together: diffs and row bases, then palette words, then the stores.
+    constexpr uint32_t kGroupsOfRows = 2 * kTileDim / 8;
+    constexpr uint32_t kSteps = (kGroupsOfRows + kTileFusedWaves - 1) / kTileFusedWaves;  // 3
+    const uint32_t sub = lane >> 3, xb = 4 * (lane & 7);
+    uint32_t blk[kSteps], dw[kSteps];
+    int32_t base[kSteps];
+#pragma unroll
+    for (uint32_t i = 0; i < kSteps; ++i) {
+      const uint32_t grp0 = wave + kTileFusedWaves * i;
+      const uint32_t q = 8 * (grp0 < kGroupsOfRows ? grp0 : wave) + sub;  // a spare step repeats group `wave`
+      const uint32_t P = q / kTileDim, y = q % kTileDim;
+      const uint32_t ty = P ? ty1 : ty0, tx = P ? tx1 : tx0;
+      const uint32_t row0 = (ty * kTileDim + y) * bx + tx * kTileDim;
+      blk[i] = row0 + xb;
+      dw[i] = *(GPtr<const uint32_t>)(diffs + blk[i]);
+      const uint32_t g = row0 / kGroupSyms;
+      int32_t before;
+      if (few_groups) {
+        before = __shfl(gpre, static_cast<int>(g - g_min));
+      } else {
+        before = 0;
+        for (uint32_t k = xb / 4; k < g; k += 8) before += d.group_sums[k];
+#pragma unroll
+        for (int off = 4; off > 0; off >>= 1) before += __shfl_xor(before, off, 8);
+      }
+      base[i] = before + GroupLocalPrefix(d, row0);
+    }
+    uint32_t word[kSteps][4];
+#pragma unroll
+    for (uint32_t i = 0; i < kSteps; ++i) {
+      int32_t run[4];
+      int32_t acc = 0;
+#pragma unroll
+      for (uint32_t k = 0; k < 4; ++k) {
+        acc += static_cast<int32_t>((dw[i] >> (8 * k)) & 0xFFu) - 128;
+        run[k] = acc;
+      }
+      int32_t incl = acc;
+#pragma unroll
+      for (uint32_t off = 1; off < 8; off <<= 1) {
+        const int32_t n = __shfl_up(incl, off, 8);
+        if ((lane & 7) >= off) incl += n;
+      }
+      const int32_t start = base[i] + incl - acc;
+#pragma unroll
+      for (uint32_t k = 0; k < 4; ++k) {
+        const uint32_t pi = static_cast<uint32_t>(start + run[k]);
+        word[i][k] = 0;
+        if (pi < words) word[i][k] = pal[pi]; else bad = true;
+      }
+    }
+#pragma unroll
+    for (uint32_t i = 0; i < kSteps; ++i) {
+      const uint32_t grp = wave + kTileFusedWaves * i;
+      if (grp >= kGroupsOfRows) continue;
+      const uint32_t q = 8 * grp + sub;
+      const uint32_t P = q / kTileDim, y = q % kTileDim;
+      uint32_t pw[6];
+#pragma unroll
+      for (uint32_t p = 0; p < 6; ++p)
+        pw[p] = *reinterpret_cast<const uint32_t *>(tiles + (p >> 1) * kWaveLdsBytes +
+                                                    ((p & 1) * 2 + P) * kTileSyms + y * kTileDim + xb);
+      uint32_t o[8];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        o[2 * k] = Rgb565(SByte(pw[0], k), SByte(pw[2], k), SByte(pw[3], k)) |
+                   (Rgb565(SByte(pw[1], k), SByte(pw[4], k), SByte(pw[5], k)) << 16);
+        o[2 * k + 1] = word[i][k];
+      }
+      const GPtr<uint4> dst = (GPtr<uint4>)(d.out + 8 * static_cast<size_t>(blk[i]));
+      dst[0] = make_uint4(o[0], o[1], o[2], o[3]);
+      dst[1] = make_uint4(o[4], o[5], o[6], o[7]);
+    }
+    if (bad) atomicOr(a.err, kErrPaletteIndex);
+    LdsBarrier();  // the next unit's transposes reuse the tiles' space
+  }
+}
+
 __global__ __launch_bounds__(256) void AssembleRgbKernel(KernelArgs a) {
   for (uint32_t u = blockIdx.x;; u += gridDim.x) {
     const Located loc = Locate(a, kStageAssemble, u, 1);
@@ -1166,10 +1331,11 @@ hipError_t LaunchStage(uint32_t stage, const KernelArgs &args, uint32_t grid,
       return hipErrorInvalidValue;
     }
     case kStageIndices:
-      if (cfg.fused_waves) return hipSuccess;  // the assembly scans the indices itself
+      if (cfg.fused_waves || cfg.tile_fused) return hipSuccess;  // the assembly scans them itself
       hipLaunchKernelGGL(DecodeIndicesKernel, dim3(grid), dim3(256), 0, stream, args);
       return hipGetLastError();
     case kStageWavelet:
+      if (cfg.tile_fused) return hipSuccess;  // inside the assembly
       if (cfg.wavelet_waves == 1)
         hipLaunchKernelGGL(InvWaveletKernel<1>, dim3(grid), dim3(64), 0, stream, args);
       else if (cfg.wavelet_waves == 2)
@@ -1180,7 +1346,10 @@ hipError_t LaunchStage(uint32_t stage, const KernelArgs &args, uint32_t grid,
         return hipErrorInvalidValue;
       return hipGetLastError();
     case kStageAssemble:
-      if (cfg.rgb) {
+      if (cfg.tile_fused) {
+        if (cfg.rgb) return hipErrorInvalidValue;  // the fused kernel writes DXT1 only
+        hipLaunchKernelGGL(WaveletAssembleKernel, dim3(grid), dim3(kTileFusedWaves * 64), 0, stream, args);
+      } else if (cfg.rgb) {
         if (cfg.fused_waves) return hipErrorInvalidValue;  // the fused scan writes DXT1 only
         hipLaunchKernelGGL(AssembleRgbKernel, dim3(grid), dim3(256), 0, stream, args);
       } else if (cfg.fused_waves && cfg.fused_wide) {

@@ -185,6 +185,8 @@ typedef struct gst_batch_config {
                                        builds each table in LDS (4 or 8 waves per group) */
   uint32_t assembly_wide;           /* fused assembly with 16 consecutive blocks per lane
                                        (one scan per 1024 blocks) instead of 4 x 4 */
+  uint32_t separate_wavelet;        /* 1: the inverse wavelet as its own launch; 0: it runs
+                                       inside the DXT1 assembly (planes stay in LDS) */
 } gst_batch_config;
 
 int gst_batch_create(const gst_batch_config *cfg, gst_batch **out);
@@ -217,12 +219,15 @@ typedef struct gst_tune_result {
   uint32_t candidates;               /* shapes timed */
   float stage_ms[5];                 /* median time of the chosen shape per stage */
   float total_ms;
+  uint32_t wavelet_fused;            /* 1: wavelet + index scan + DXT1 assembly in one kernel */
 } gst_tune_result;
 /* No reference counterpart.  Times every launch shape of every stage on the bound
  * batch (median of `reps` launches each, 0 = 5) and keeps the fastest per stage:
  * rANS streams per wavefront {32, 64} x waves per work-group {1, 2, 4, 8} x grid,
  * wavelet waves per work-group {1, 2, 4} x grid, and the index scan + assembly either
- * fused (waves per work-group {1, 2, 4} x grid) or as two launches (grid each).  The
+ * fused (waves per work-group {1, 2, 4} x grid) or as two launches (grid each); then
+ * the wavelet inside the DXT1 assembly (grid) replaces the last three stages when it
+ * beats their sum.  The
  * bound images are decoded along the way.  Drops a captured graph
  * (capture again afterwards).  Blocking. */
 int gst_batch_autotune(gst_batch *b, gst_stream stream, uint32_t reps, gst_tune_result *out);

@@ -46,17 +46,19 @@ def test_uneven_batch_matches_oracle(device):
         b.close()
 
 
-@pytest.mark.parametrize("gpw,waves,wwaves,awaves,separate,sep_tables,wide", [
-    (2, 4, 1, 4, False, False, False), (1, 4, 2, 1, False, True, True), (2, 8, 4, 2, False, False, False),
-    (1, 8, 1, 4, True, False, False), (2, 1, 1, 0, True, False, False), (1, 2, 4, 0, True, True, False),
-    (2, 4, 2, 2, False, False, True), (2, 8, 1, 4, False, True, True)])
-def test_launch_shapes_agree(device, gpw, waves, wwaves, awaves, separate, sep_tables, wide):
+@pytest.mark.parametrize("gpw,waves,wwaves,awaves,separate,sep_tables,wide,sep_wavelet", [
+    (2, 4, 1, 4, False, False, False, True), (1, 4, 2, 1, False, True, True, True),
+    (2, 8, 4, 2, False, False, False, True), (1, 8, 1, 4, True, False, False, True),
+    (2, 1, 1, 0, True, False, False, True), (1, 2, 4, 0, True, True, False, True),
+    (2, 4, 2, 2, False, False, True, True), (2, 8, 1, 4, False, True, True, True),
+    (2, 4, 1, 4, False, False, False, False), (1, 8, 1, 4, False, True, False, False)])
+def test_launch_shapes_agree(device, gpw, waves, wwaves, awaves, separate, sep_tables, wide, sep_wavelet):
     files, dxts = encode.make_textures(MIXED[:4])
     blocks, pal = _capacity(files)
     b = gst_amd.Batch(len(files), blocks, pal, groups_per_wave=gpw, waves_per_group=waves,
                       grids={"ans_decode": 64, "inv_wavelet": 128, "decode_indices": 16, "assemble": 32},
                       wavelet_waves=wwaves, assembly_waves=awaves, separate_index_scan=separate,
-                      separate_table_stage=sep_tables, assembly_wide=wide)
+                      separate_table_stage=sep_tables, assembly_wide=wide, separate_wavelet=sep_wavelet)
     try:
         _, _, d_outs = _bind(device, b, files)
         b.decode(device.stream)
@@ -137,8 +139,10 @@ def test_autotune_keeps_results_bit_exact(device):
         assert tuned["streams_per_wavefront"] in (32, 64)
         assert tuned["wavelet_waves_per_group"] in (1, 2, 4)
         assert tuned["candidates"] >= 7 * 5 + 3 * 5 + 2 * 5
-        # A fused index scan and tables built in LDS have no launch of their own.
-        folded = {"decode_indices": tuned["assembly_waves_per_group"] > 0,
+        # A fused index scan, a wavelet inside the assembly and tables built in LDS
+        # have no launch of their own.
+        folded = {"decode_indices": tuned["assembly_waves_per_group"] > 0 or tuned["wavelet_fused"] == 1,
+                  "inv_wavelet": tuned["wavelet_fused"] == 1,
                   "build_table": tuned["ans_builds_tables"] == 1}
         for k, v in tuned["stage_ms"].items():
             assert v == 0 if folded.get(k) else v > 0, (k, v)
@@ -154,3 +158,25 @@ def test_autotune_keeps_results_bit_exact(device):
         assert b.error_bits(device.stream) == 0
     finally:
         b.close()
+
+
+@pytest.mark.parametrize("shapes", [
+    # 3 tiles per tile row: work-group pairs of tile positions straddle tile rows.
+    [(501, 384, 1024), (502, 640, 1024)],
+    # 528 tiles per tile row: a pair's rows touch more than 64 index-stream groups.
+    [(503, 67584, 128)],
+])
+def test_wavelet_inside_assembly_on_odd_and_wide_tilings(device, shapes):
+    files, dxts = encode.make_textures(shapes)
+    blocks, pal = _capacity(files)
+    fused = gst_amd.Batch(len(files), blocks, pal, grids={"assemble": 7})
+    separate = gst_amd.Batch(len(files), blocks, pal, separate_wavelet=True)
+    try:
+        for b in (fused, separate):
+            _, _, outs = _bind(device, b, files)
+            b.decode(device.stream)
+            assert [device.download(o) for o in outs] == dxts
+            assert b.error_bits(device.stream) == 0
+    finally:
+        fused.close()
+        separate.close()
