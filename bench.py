@@ -41,20 +41,25 @@ def corpus_c3(n=128, size_seed=42):
     return [(1000 + i, SIZES_C3[p][0], SIZES_C3[p][1]) for i, p in enumerate(picks)]
 
 
-def stage_bytes(hdrs, fused=False):
-    """Algorithmic read+write bytes per stage for one decode of `hdrs`.  With the index
-    scan fused into the assembly, the assembly reads the nv diff bytes (plus nv/64
-    bytes of lane totals) instead of nv int32 indices, and the scan stage moves nothing."""
+def stage_bytes(hdrs, mode="tile_fused"):
+    """Algorithmic read+write bytes per stage for one decode of `hdrs`.  `mode` is how
+    the last stages run: "tile_fused" (wavelet, index scan and DXT1 assembly in one
+    kernel: the int8 planes stay in LDS), "fused" (index scan inside the assembly:
+    it reads the nv diff bytes instead of nv int32 indices) or "separate".  The rANS
+    stage also writes the index stream's partial sums (nv/64 + nv/8 bytes), which the
+    fused assemblies read."""
     nv = sum(h.num_blocks for h in hdrs)
     P = sum(h.palette_bytes for h in hdrs)
     S = sum(h.stream_bytes for h in hdrs)
     n = len(hdrs)
+    partial = nv // 64 + nv // 8
+    scan_folded = mode in ("tile_fused", "fused")
     return {
         "build_table": 2048 * n + 32768 * n,
-        "ans_decode": (S + 32768 * n) + (7 * nv + P) + (nv // 64 if fused else 0),
-        "decode_indices": 0 if fused else nv + 4 * nv,
-        "inv_wavelet": 6 * nv + 6 * nv,
-        "assemble": (6 * nv + nv + nv // 64 + P) + 8 * nv if fused else (6 * nv + 4 * nv + P) + 8 * nv,
+        "ans_decode": (S + 32768 * n) + (7 * nv + P) + partial,
+        "decode_indices": 0 if scan_folded else nv + 4 * nv,
+        "inv_wavelet": 0 if mode == "tile_fused" else 6 * nv + 6 * nv,
+        "assemble": ((6 * nv + nv + partial + P) if scan_folded else (6 * nv + 4 * nv + P)) + 8 * nv,
     }
 
 
@@ -70,8 +75,11 @@ def measured_traffic(stage, n_images, requested):
     return None if per is None else per["read"] + per["write"]
 
 
-def pipeline_bytes(hdrs):
-    b_pipe = sum((h.stream_bytes + 2048) + 42 * h.num_blocks + 2 * h.palette_bytes + 65536 for h in hdrs)
+def pipeline_bytes(hdrs, mode="tile_fused", tables_in_lds=True):
+    """B_pipe: the stages' algorithmic bytes in `mode` (the table stage's only when it
+    has a launch of its own); B_min: inputs read once (streams, frequencies) and the
+    DXT1 output written once."""
+    b_pipe = sum(v for k, v in stage_bytes(hdrs, mode).items() if k != "build_table" or not tables_in_lds)
     b_min = sum((h.stream_bytes + 2048) + 8 * h.num_blocks for h in hdrs)
     return b_pipe, b_min
 
@@ -342,11 +350,14 @@ def main():
     value = texels / (ms * 1e-3) / 1e6
     log("batched: %.3f ms/step  %.0f Mtexel/s  stages %s" % (ms, value, stage))
 
-    fused = tuned["assembly_waves_per_group"] > 0 if tuned else True  # the batch default is fused
-    sb = stage_bytes(hdrs, fused)
+    if tuned is None or tuned["wavelet_fused"]:  # the batch default runs the wavelet in the assembly
+        mode = "tile_fused"
+    else:
+        mode = "fused" if tuned["assembly_waves_per_group"] > 0 else "separate"
+    sb = stage_bytes(hdrs, mode)
     dom = max(stage, key=stage.get)
     achieved = sb[dom] / (stage[dom] * 1e-3) / 1e9
-    b_pipe, b_min = pipeline_bytes(hdrs)
+    b_pipe, b_min = pipeline_bytes(hdrs, mode, tables_in_lds=tuned is None or tuned["ans_builds_tables"] == 1)
     result = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -390,7 +401,9 @@ def main():
         "launch_ms_per_step": {k: round(v, 4) for k, v in launch.items()},
         "stage_ms": {k: round(v, 4) for k, v in stage.items()},
         "stage_gbs": {k: round(sb[k] / (stage[k] * 1e-3) / 1e9, 1) if sb[k] else None for k in stage},
-        "index_scan": "fused into the assembly" if fused else "separate launch",
+        "last_stages": {"tile_fused": "wavelet, index scan and DXT1 assembly in one kernel",
+                        "fused": "wavelet launch, then index scan inside the assembly",
+                        "separate": "wavelet, index scan and assembly launches"}[mode],
     }
 
     if not args.no_plain:
