@@ -180,3 +180,37 @@ def test_wavelet_inside_assembly_on_odd_and_wide_tilings(device, shapes):
     finally:
         fused.close()
         separate.close()
+
+
+@pytest.mark.parametrize("capacity", [255, 256, 300, 5000])
+def test_large_plan_capacities_locate_units(device, capacity):
+    # Plans of more than 255 images find a unit's image with a 64-way narrowing per
+    # round trip instead of one load of all prefix entries; 255 and 256 straddle the
+    # switch.  Bound batches smaller than the capacity leave padded prefix entries.
+    files, dxts = encode.make_textures(MIXED[:3] + [(601, 512, 256)] * 2)
+    blocks, pal = _capacity(files)
+    b = gst_amd.Batch(capacity, blocks, pal)
+    try:
+        _, _, outs = _bind(device, b, files)
+        b.decode(device.stream)
+        assert [device.download(o) for o in outs] == dxts
+        assert b.error_bits(device.stream) == 0
+    finally:
+        b.close()
+
+
+def test_full_plan_of_many_small_images(device):
+    # 300 bound images: every prefix entry is real and the narrowing path runs to the
+    # last image.
+    specs = [(700 + i, 512, 256) for i in range(300)]
+    files, dxts = encode.make_textures(specs)
+    blocks, pal = _capacity(files)
+    b = gst_amd.Batch(len(files), blocks, pal)
+    try:
+        _, _, outs = _bind(device, b, files)
+        b.decode(device.stream)
+        got = [device.download(o) for o in outs]
+        assert got == dxts
+        assert b.error_bits(device.stream) == 0
+    finally:
+        b.close()
