@@ -101,10 +101,15 @@ struct Slot {
   std::vector<std::shared_ptr<Req>> reqs;
 };
 
+// Algorithmic bytes of one image through the batch's default two kernels: the rANS
+// stage reads the streams and the four 8 KiB tables' frequencies and writes the
+// symbols and the index stream's partial sums; the fused wavelet + assembly reads the
+// symbols and partial sums and writes the DXT1 blocks.
 uint64_t PipelineBytes(const gst_header &h) {
   const uint64_t nv = static_cast<uint64_t>(h.width) * h.height / 16;
-  const uint64_t C = 2048ull + h.y_cmp_sz + h.chroma_cmp_sz + h.palette_sz + h.indices_sz;
-  return C + 42 * nv + 2ull * h.palette_bytes + 65536;
+  const uint64_t S = static_cast<uint64_t>(h.y_cmp_sz) + h.chroma_cmp_sz + h.palette_sz + h.indices_sz;
+  const uint64_t partial = nv / 64 + nv / 8;
+  return (S + 32768) + (7 * nv + h.palette_bytes + partial) + (7 * nv + h.palette_bytes + partial) + 8 * nv;
 }
 
 }  // namespace
@@ -223,11 +228,15 @@ int LaunchBatch(gst_server &s, Slot &slot, std::vector<std::shared_ptr<Req>> &&r
     }
     rc = gst_batch_bind(slot.batch, hdrs.data(), static_cast<uint32_t>(hdrs.size()), files.data(),
                         outs.data(), st);
-    if (rc == GST_OK && !slot.graph_ready) {
-      rc = gst_batch_capture_graph(slot.batch, st);
-      slot.graph_ready = rc == GST_OK;
+    if (!s.cfg.direct_launch) {
+      if (rc == GST_OK && !slot.graph_ready) {
+        rc = gst_batch_capture_graph(slot.batch, st);
+        slot.graph_ready = rc == GST_OK;
+      }
+      if (rc == GST_OK) rc = gst_batch_launch_graph(slot.batch, st);
+    } else if (rc == GST_OK) {
+      rc = gst_batch_decode(slot.batch, st);
     }
-    if (rc == GST_OK) rc = gst_batch_launch_graph(slot.batch, st);
     if (rc == GST_OK) rc = gst_batch_error_bits_async(slot.batch, st, slot.err_host);
     if (rc != GST_OK) return rc;
   }

@@ -37,6 +37,10 @@ typedef struct gst_server_config {
   uint32_t dry_run;             /* 1: no GPU work; decodes complete at once (scheduler tests) */
   uint32_t ans_groups_per_wave; /* batch plan launch shape (0: defaults) */
   uint32_t ans_waves_per_group;
+  uint32_t direct_launch;       /* 0: replay each slot's captured hipGraph (one call per
+                                   batch for the scheduler thread); 1: launch the stage kernels
+                                   on the slot's stream (lower p50 at light load, more host
+                                   time per batch) */
 } gst_server_config;
 
 typedef struct gst_completion {

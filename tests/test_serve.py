@@ -106,9 +106,10 @@ def test_poisson_load_generator_dry_run():
 # ---- GPU ---------------------------------------------------------------------------------
 
 @pytest.mark.gpu
-def test_served_results_equal_alone_and_in_the_mix(device):
+@pytest.mark.parametrize("direct_launch", [False, True])
+def test_served_results_equal_alone_and_in_the_mix(device, direct_launch):
     """Each request decoded alone (one at a time) and in a concurrent stream of
-    arrivals gives the same bit-exact blocks."""
+    arrivals gives the same bit-exact blocks, with graph replays or stream launches."""
     torch = device.torch
     specs = [(501, 512, 512), (502, 1024, 1024), (503, 512, 256), (504, 2048, 2048), (505, 768, 512),
              (506, 512, 1024)]
@@ -126,7 +127,7 @@ def test_served_results_equal_alone_and_in_the_mix(device):
         return host.tobytes()
 
     alone = {}
-    with Server(1 << 30, auto_release=False) as s:
+    with Server(1 << 30, auto_release=False, direct_launch=direct_launch) as s:
         for i, f in enumerate(files):
             s.submit(f, tag=i)
             s.drain()
@@ -136,7 +137,7 @@ def test_served_results_equal_alone_and_in_the_mix(device):
             s.release(c.ticket)
     assert [alone[i] for i in range(len(files))] == dxts
 
-    with Server(1 << 30, auto_release=False, slots=3) as s:
+    with Server(1 << 30, auto_release=False, slots=3, direct_launch=direct_launch) as s:
         order = list(range(len(files))) * 4
         np.random.default_rng(0).shuffle(order)
         for i in order:
