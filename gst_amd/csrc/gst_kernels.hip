@@ -360,9 +360,11 @@ __device__ __forceinline__ void DecodeHalfGroups(const uint32_t *tab, uint16_t *
         pB -= thi;
         if (active && x < kStateL) {
           // Word index p_half + rank: mbcnt over both halves counts the lower half's
-          // bits for upper lanes, so upper lanes start from pB - tlo.
-          const uint32_t b = pA + (upper & (pB - tlo - pA));
-          const uint32_t widx = __builtin_amdgcn_mbcnt_hi(mhi, __builtin_amdgcn_mbcnt_lo(mlo, b));
+          // bits for upper lanes, so upper lanes start from pB - tlo.  The rank needs
+          // only the ballot, so it runs beside the scalar pointer updates; one
+          // three-operand add joins them.
+          const uint32_t rank = __builtin_amdgcn_mbcnt_hi(mhi, __builtin_amdgcn_mbcnt_lo(mlo, 0u));
+          const uint32_t widx = rank + pA + (upper & (pB - tlo - pA));
           x = (x << 16) | ring_half[widx & (kRingWords - 1)];
         }
         // Step 8*blk + s lands at byte 63 - 8*blk - s of the sector.  The symbol (byte
@@ -1078,6 +1080,28 @@ __global__ __launch_bounds__(WAVES * 64) void AssembleFusedWideKernel(KernelArgs
   }
 }
 
+// Byte k of a and of b, sign-extended into the low and high 16-bit halves.
+__device__ __forceinline__ s16x2 SBytePair(uint32_t a, uint32_t b, uint32_t k) {
+  // Selector codes 0-3 take bytes of the second operand, 4-7 of the first, 0x0c a zero:
+  // a's byte to bits 8-15, b's to bits 24-31, then an arithmetic shift of each half.
+  const uint32_t sel = 0x0C | (k << 8) | (0x0Cu << 16) | ((4u + k) << 24);
+  return AsPair(__builtin_amdgcn_perm(b, a, sel)) >> static_cast<s16x2>(8);
+}
+
+// Rgb565 of both endpoints of a block at once, endpoint 1 in the low and endpoint 2 in
+// the high half: the same truncating divisions and the same low 16 bits of the shifts
+// (|y|, |co|, |cg| <= 128, so every intermediate fits 16 bits).  Arguments: the four
+// blocks' plane words (y1, y2, co1, co2, cg1, cg2) and the block's byte k.
+__device__ __forceinline__ uint32_t Rgb565Pair(uint32_t y1, uint32_t y2, uint32_t co1, uint32_t co2,
+                                               uint32_t cg1, uint32_t cg2, uint32_t k) {
+  const s16x2 y = SBytePair(y1, y2, k), co = SBytePair(co1, co2, k), cg = SBytePair(cg1, cg2, k);
+  const s16x2 t = y - TruncDiv2(cg);
+  const s16x2 g = cg + t;
+  const s16x2 b = TruncDiv2(t - co);
+  const s16x2 r = b + co;
+  return Bits(r << static_cast<s16x2>(11)) | Bits(g << static_cast<s16x2>(5)) | Bits(b);
+}
+
 // Stages 4 and 5 in one kernel: inverse wavelet, index scan and DXT1 assembly of two
 // tile positions per work-group, so the int8 planes never leave LDS (the planes
 // buffer's write and read, 12 bytes a block, are the traffic this saves).
@@ -1220,9 +1244,8 @@ __global__ __launch_bounds__(kTileFusedWaves * 64, 5) void WaveletAssembleKernel
                                                     ((p & 1) * 2 + P) * kTileSyms + y * kTileDim + xb);
       uint32_t o[8];
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        o[2 * k] = Rgb565(SByte(pw[0], k), SByte(pw[2], k), SByte(pw[3], k)) |
-                   (Rgb565(SByte(pw[1], k), SByte(pw[4], k), SByte(pw[5], k)) << 16);
+      for (uint32_t k = 0; k < 4; ++k) {
+        o[2 * k] = Rgb565Pair(pw[0], pw[1], pw[2], pw[4], pw[3], pw[5], k);
         o[2 * k + 1] = word[i][k];
       }
       const GPtr<uint4> dst = (GPtr<uint4>)(d.out + 8 * static_cast<size_t>(blk[i]));
