@@ -292,9 +292,40 @@ __device__ __forceinline__ AnsRing RequestRing(__amdgpu_buffer_rsrc_t rs, uint32
   return r;
 }
 
+// Stream-tail touch.  Every refill waits for all loads in flight, so a refill whose
+// chunk is still on its way from HBM stalls the wave for the whole round trip, a dozen
+// times per group when the compressed streams are not in any cache (the usual case in a
+// decode pipeline, whose other kernel moves a few hundred MB between two rANS passes).
+// One pass of 4-byte LDS-DMA loads, a lane per 64 bytes, pulls the rest of the half's
+// stream (bytes [lo, 512 (top - 3)), below the four chunks already requested) into L2
+// and the memory-side cache at the start; the refills then find their chunks there.
+// The loads write a 256-byte LDS sink that nothing reads and hold no VGPR; offsets
+// past the stream (and halves with nothing left) read nothing through the buffer's
+// range check.  A group's stream is at most 2 * kGroupSyms bytes, so at most
+// kTouchPasses passes of 32 lanes x 64 bytes.
+constexpr uint32_t kTouchStride = 64;
+constexpr uint32_t kTouchPasses = 2 * kGroupSyms / (kLanesPerGroup * kTouchStride);
+
+__device__ __forceinline__ void TouchStreamTail(__amdgpu_buffer_rsrc_t rs, uint32_t top, uint32_t lo_byte,
+                                                uint32_t j, uint32_t *sink) {
+  const uint32_t hi = top >= 3 ? 2 * kChunkWords * (top - 3) : 0u;
+  const uint32_t span = kLanesPerGroup * kTouchStride;
+  const uint32_t need = hi > lo_byte ? (hi - lo_byte + span - 1) / span : 0u;
+  const uint32_t n0 = static_cast<uint32_t>(__builtin_amdgcn_readlane(need, 0));
+  const uint32_t n1 = static_cast<uint32_t>(__builtin_amdgcn_readlane(need, 32));
+  const uint32_t n = min(n0 > n1 ? n0 : n1, kTouchPasses);
+  for (uint32_t i = 0; i < n; ++i) {
+    const uint32_t back = kTouchStride * (j + 1) + span * i;
+    const bool in = hi >= back && hi - back + kTouchStride > lo_byte;
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (__attribute__((address_space(3))) void *)sink, 4,
+                                             static_cast<int>(in ? hi - back : 0x7FFFFFF0u), 0, 0, 0);
+  }
+}
+
 template <bool kSum, uint32_t GPW>
 __device__ __forceinline__ void DecodeHalfGroups(const uint32_t *tab, uint16_t *ring_wave, uint16_t *ring_half,
-                                                 __amdgpu_buffer_rsrc_t rs, AnsRing ring, uint32_t lane,
+                                                 uint32_t *sink, __amdgpu_buffer_rsrc_t rs, AnsRing ring,
+                                                 uint32_t lane,
                                                  bool active, uint32_t x, uint32_t pA, uint32_t pB,
                                                  uint32_t loA, uint32_t loB, GPtr<uint8_t> outp,
                                                  GPtr<int32_t> group_sum, GPtr<int32_t> lane_sum,
@@ -307,10 +338,11 @@ __device__ __forceinline__ void DecodeHalfGroups(const uint32_t *tab, uint16_t *
   uint16_t *const ringB = ring_wave + kRingWords;
   uint32_t topA = ring.topA, topB = ring.topB;
   // Chunks top-2 and top-3 go in flight in two fixed registers by chunk parity (even
-  // chunks in pXE, odd in pXO).  A load lands in the register its chunk is later
-  // stored from, so the wait before that store covers only that load (register moves
-  // between in-flight loads made the compiler wait for every outstanding load and
-  // store, sector stores included).
+  // chunks in pXE, odd in pXO): a load lands in the register its chunk is later
+  // stored from, so no register moves sit between in-flight loads.  The wait before a
+  // refill's store is still vmcnt(0) (the number of refill loads between two crossings
+  // is data-dependent and output stores are in flight), which TouchStreamTail makes
+  // cheap.
   u32x2 pAE = LoadChunk(rs, (topA & 1u) ? topA - 3 : topA - 2, lane);
   u32x2 pAO = LoadChunk(rs, (topA & 1u) ? topA - 2 : topA - 3, lane);
   u32x2 pBE = {0u, 0u}, pBO = {0u, 0u};
@@ -318,6 +350,7 @@ __device__ __forceinline__ void DecodeHalfGroups(const uint32_t *tab, uint16_t *
     pBE = LoadChunk(rs, (topB & 1u) ? topB - 3 : topB - 2, lane);
     pBO = LoadChunk(rs, (topB & 1u) ? topB - 2 : topB - 3, lane);
   }
+  TouchStreamTail(rs, lane < 32 ? topA : topB, 2u * (lane < 32 ? loA : loB), j, sink);
   StoreChunk(ringA, topA, lane, ring.rA0);
   StoreChunk(ringA, topA - 1, lane, ring.rA1);
   if (GPW == 2) {
@@ -538,6 +571,7 @@ void AnsDecodeKernel(KernelArgs a) {
   __shared__ uint32_t build_cum[kBuild ? 256 : 1];
   __shared__ uint32_t build_freq[kBuild ? 256 : 1];
   __shared__ uint32_t build_wsum[4];
+  __shared__ uint32_t touch_sink[64];
   constexpr uint32_t kGroupsPerUnit = WAVES * GPW;
   const uint32_t lane = threadIdx.x & 63, wave = Uniform(threadIdx.x >> 6);
   const uint32_t half = GPW == 2 ? (lane >> 5) : 0;
@@ -600,15 +634,15 @@ void AnsDecodeKernel(KernelArgs a) {
     const GPtr<uint8_t> outp = d.symbols + SymbolOffset(d, k) + static_cast<size_t>(g) * kGroupSyms + j * kSymsPerLane;
     uint16_t *ring_half = ring[wave][half];
     if (k == 3)
-      DecodeHalfGroups<true, GPW>(tab, ring[wave][0], ring_half, rs, chunks, lane, active, x, pA, pB, loA, loB,
-                                  outp, d.group_sums + g,
+      DecodeHalfGroups<true, GPW>(tab, ring[wave][0], ring_half, touch_sink, rs, chunks, lane, active, x, pA, pB,
+                                  loA, loB, outp, d.group_sums + g,
                                   d.lane_sums ? d.lane_sums + g * kLanesPerGroup : nullptr,
                                   d.lane_sums ? d.lane_sums + LaneSuffixBase(d.nv) + g * kLanesPerGroup * 8
                                               : nullptr,
                                   a.err);
     else
-      DecodeHalfGroups<false, GPW>(tab, ring[wave][0], ring_half, rs, chunks, lane, active, x, pA, pB, loA, loB,
-                                   outp, nullptr, nullptr, nullptr, a.err);
+      DecodeHalfGroups<false, GPW>(tab, ring[wave][0], ring_half, touch_sink, rs, chunks, lane, active, x, pA, pB,
+                                   loA, loB, outp, nullptr, nullptr, nullptr, a.err);
   }
 }
 
