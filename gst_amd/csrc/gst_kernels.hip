@@ -395,7 +395,9 @@ __device__ __forceinline__ void DecodeHalfGroups(const uint32_t *tab, uint16_t *
           // Word index p_half + rank: mbcnt over both halves counts the lower half's
           // bits for upper lanes, so upper lanes start from pB - tlo.  The rank needs
           // only the ballot, so it runs beside the scalar pointer updates; one
-          // three-operand add joins them.
+          // three-operand add joins them.  (Pointers kept mod 512 in SGPRs, with a
+          // mirrored ring tail, save the mask but lengthen the scalar chain ahead of
+          // this read, and the stage got slower.)
           const uint32_t rank = __builtin_amdgcn_mbcnt_hi(mhi, __builtin_amdgcn_mbcnt_lo(mlo, 0u));
           const uint32_t widx = rank + pA + (upper & (pB - tlo - pA));
           x = (x << 16) | ring_half[widx & (kRingWords - 1)];
@@ -727,10 +729,12 @@ typedef uint16_t u16x2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ s16x2 AsPair(uint32_t w) { return __builtin_bit_cast(s16x2, w); }
 __device__ __forceinline__ uint32_t Bits(s16x2 v) { return __builtin_bit_cast(uint32_t, v); }
 
-// C's truncating division of each half: negative values take divisor - 1 first.
+// C's truncating division of each half: negative values take divisor - 1 first.  Every
+// sum divided here is within 2 * 3498 + 2 < 2^14 in magnitude (bound above), so bits
+// 14 and 15 agree and the top two bits are the divisor - 1 (3) exactly when negative.
 __device__ __forceinline__ s16x2 TruncDiv4(s16x2 s) {
-  const s16x2 neg = s >> static_cast<s16x2>(15);  // 0 or -1
-  return (s + neg * static_cast<s16x2>(-3)) >> static_cast<s16x2>(2);
+  const u16x2 fix = __builtin_bit_cast(u16x2, s) >> static_cast<u16x2>(14);  // 0 or 3
+  return (s + __builtin_bit_cast(s16x2, fix)) >> static_cast<s16x2>(2);
 }
 __device__ __forceinline__ s16x2 TruncDiv2(s16x2 s) {
   const u16x2 sign = __builtin_bit_cast(u16x2, s) >> static_cast<u16x2>(15);  // 0 or 1
