@@ -1132,7 +1132,10 @@ __device__ __forceinline__ s16x2 SBytePair(uint32_t a, uint32_t b, uint32_t k) {
 // blocks' plane words (y1, y2, co1, co2, cg1, cg2) and the block's byte k.
 __device__ __forceinline__ uint32_t Rgb565Pair(uint32_t y1, uint32_t y2, uint32_t co1, uint32_t co2,
                                                uint32_t cg1, uint32_t cg2, uint32_t k) {
-  const s16x2 y = SBytePair(y1, y2, k), co = SBytePair(co1, co2, k), cg = SBytePair(cg1, cg2, k);
+  const s16x2 y = SBytePair(y1, y2, k), co = SBytePair(co1, co2, k);
+  uint32_t cgw = Bits(SBytePair(cg1, cg2, k));
+  asm volatile("" : "+v"(cgw));  // else cg / 2 is split into two 16-bit halves and re-packed
+  const s16x2 cg = AsPair(cgw);
   const s16x2 t = y - TruncDiv2(cg);
   const s16x2 g = cg + t;
   const s16x2 b = TruncDiv2(t - co);
