@@ -391,16 +391,17 @@ __device__ __forceinline__ void DecodeHalfGroups(const uint32_t *tab, uint16_t *
         const uint32_t tlo = __popc(mlo), thi = __popc(mhi);
         pA -= tlo;
         pB -= thi;
-        if (active && x < kStateL) {
+        {
           // Word index p_half + rank: mbcnt over both halves counts the lower half's
           // bits for upper lanes, so upper lanes start from pB - tlo.  The rank needs
           // only the ballot, so it runs beside the scalar pointer updates; one
-          // three-operand add joins them.  (Pointers kept mod 512 in SGPRs, with a
-          // mirrored ring tail, save the mask but lengthen the scalar chain ahead of
-          // this read, and the stage got slower.)
+          // three-operand add joins them.  Every lane reads (a lane that keeps its
+          // state reads a word of the ring it never uses) and a select on the ballot
+          // keeps the new state: no exec-mask round trip through the scalar unit.
           const uint32_t rank = __builtin_amdgcn_mbcnt_hi(mhi, __builtin_amdgcn_mbcnt_lo(mlo, 0u));
           const uint32_t widx = rank + pA + (upper & (pB - tlo - pA));
-          x = (x << 16) | ring_half[widx & (kRingWords - 1)];
+          const uint32_t xn = (x << 16) | ring_half[widx & (kRingWords - 1)];
+          x = (active && x < kStateL) ? xn : x;
         }
         // Step 8*blk + s lands at byte 63 - 8*blk - s of the sector.  The symbol (byte
         // 3 of fs) goes in with v_perm: selector codes 4-7 take bytes of the first
