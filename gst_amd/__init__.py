@@ -140,11 +140,31 @@ def _declare(lib):
         "gst_batch_autotune": ([_P, _P, _U32, ctypes.POINTER(TuneResult)], ctypes.c_int),
         "gst_batch_device_bytes": ([_P], _SZ),
         "gst_batch_destroy": ([_P], ctypes.c_int),
+        "gst_ans_normalize_frequencies": ([ctypes.POINTER(_U32), _U32, ctypes.c_int32, ctypes.POINTER(_U32)],
+                                          ctypes.c_int),
+        "gst_ans_decoder_create": ([ctypes.POINTER(_U32), _U32, _U32, ctypes.POINTER(_P)], ctypes.c_int),
+        "gst_ans_rebuild_table": ([_P, ctypes.POINTER(_U32), _U32], ctypes.c_int),
+        "gst_ans_get_table": ([_P, _P, _P, _P], ctypes.c_int),
+        "gst_ans_decode": ([_P, ctypes.POINTER(_U32), _U32, ctypes.POINTER(_P), ctypes.POINTER(_SZ), _U32, _P],
+                           ctypes.c_int),
+        "gst_ans_decoder_destroy": ([_P], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
         fn.argtypes = args
         fn.restype = res
+
+
+def _torch_runtime_first():
+    """PyTorch-ROCm ships its own HIP and HSA runtimes.  When torch is installed it
+    loads and initialises them before libgst_amd.so is loaded, so that the library binds
+    to the runtime already in the process; loaded the other way round, the system
+    runtime holds the device and torch reports no GPU."""
+    try:
+        import torch
+        torch.cuda.is_available()
+    except Exception:  # torch absent or without GPU support: nothing to order
+        pass
 
 
 def lib():
@@ -153,6 +173,7 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise ImportError("libgst_amd.so is not built; run gst_amd/build.py (no fallback exists)")
+        _torch_runtime_first()
         handle = ctypes.CDLL(LIB_PATH)
         _declare(handle)
         _lib = handle
@@ -351,6 +372,74 @@ class Batch:
     def close(self):
         if self._h:
             lib().gst_batch_destroy(self._h)
+            self._h = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---- stand-alone interleaved rANS (ans/ans_ocl.h:26-72) -----------------------------
+
+ANS_TABLE_SIZE = 2048
+ANS_SYMBOLS_PER_STREAM = 256
+
+
+def _u32_array(values):
+    values = [int(v) for v in values]
+    return (_U32 * max(1, len(values)))(*values), len(values)
+
+
+def NormalizeFrequencies(counts, M=ANS_TABLE_SIZE):
+    """ans::GenerateHistogram(counts, M) (ans/histogram.cpp:40-121)."""
+    arr, n = _u32_array(counts)
+    out = (_U32 * max(1, n))()
+    _check(lib().gst_ans_normalize_frequencies(arr, n, int(M), out), "gst_ans_normalize_frequencies")
+    return [int(v) for v in out[:n]]
+
+
+class AnsDecoder:
+    """ans::ocl::OpenCLDecoder (ans/ans_ocl.h:26-72): one device table, blocking decodes."""
+
+    def __init__(self, F, num_interleaved):
+        arr, n = _u32_array(F)
+        self._h = _P()
+        _check(lib().gst_ans_decoder_create(arr, n, int(num_interleaved), ctypes.byref(self._h)),
+               "gst_ans_decoder_create")
+
+    def rebuild_table(self, F):
+        arr, n = _u32_array(F)
+        _check(lib().gst_ans_rebuild_table(self._h, arr, n), "gst_ans_rebuild_table")
+
+    def table(self):
+        """(symbols u8[2048], frequencies u16[2048], cumulative frequencies u16[2048])."""
+        import numpy as np
+        sym = np.zeros(ANS_TABLE_SIZE, np.uint8)
+        freq = np.zeros(ANS_TABLE_SIZE, np.uint16)
+        cum = np.zeros(ANS_TABLE_SIZE, np.uint16)
+        _check(lib().gst_ans_get_table(self._h, sym.ctypes.data, freq.ctypes.data, cum.ctypes.data),
+               "gst_ans_get_table")
+        return sym, freq, cum
+
+    def decode(self, states, data):
+        """`data`: one bytes object (all states interleaved in it) or a list of them (each
+        holding len(states) / len(data) streams).  Returns one bytes object of 256
+        symbols per state."""
+        blobs = [bytes(data)] if isinstance(data, (bytes, bytearray)) else [bytes(d) for d in data]
+        st, ns = _u32_array(states)
+        bufs = [ctypes.create_string_buffer(b, max(1, len(b))) for b in blobs]
+        ptrs = (_P * len(bufs))(*[ctypes.cast(b, _P) for b in bufs])
+        lens = (_SZ * len(bufs))(*[len(b) for b in blobs])
+        out = ctypes.create_string_buffer(ns * ANS_SYMBOLS_PER_STREAM)
+        _check(lib().gst_ans_decode(self._h, st, ns, ptrs, lens, len(bufs), out), "gst_ans_decode")
+        raw = out.raw
+        return [raw[i * ANS_SYMBOLS_PER_STREAM:(i + 1) * ANS_SYMBOLS_PER_STREAM] for i in range(ns)]
+
+    def close(self):
+        if self._h:
+            lib().gst_ans_decoder_destroy(self._h)
             self._h = _P()
 
     def __del__(self):

@@ -49,7 +49,7 @@ def _atomic_build(out, cmd_for):
 def build_decoder():
     out = os.path.join(HERE, "libgst_amd.so")
     srcs = [os.path.join(CSRC, "gst_kernels.hip"), os.path.join(CSRC, "gst_decoder.cpp"),
-            os.path.join(CSRC, "gst_serve.cpp")]
+            os.path.join(CSRC, "gst_serve.cpp"), os.path.join(CSRC, "gst_ans.cpp")]
     srcs = [s for s in srcs if os.path.exists(s)]
     return _atomic_build(out, lambda o: [HIPCC, *HIP_FLAGS, "-I" + INCLUDE, "-I" + CSRC, *srcs,
                                          "-o", o, "-lpthread"])
@@ -64,12 +64,16 @@ def build_encoder():
 
 
 def build_cpp_tests():
-    """tests/cpp/codec_test: the reference's end-to-end test against the C++ API."""
-    src = os.path.join(ROOT, "tests", "cpp", "codec_test.cpp")
-    out = os.path.join(ROOT, "tests", "cpp", "codec_test")
-    return _atomic_build(out, lambda o: [HIPCC, "--offload-arch=gfx950", "-O2", "-std=c++17",
-                                         "-Wno-unused-command-line-argument", "-I" + INCLUDE, src,
-                                         "-L" + HERE, "-lgst_amd", "-Wl,-rpath," + HERE, "-o", o])
+    """tests/cpp/codec_test and tests/cpp/ans_ocl_test: the reference's end-to-end decoder
+    test and its ANS decoder tests against the C++ API."""
+    outs = []
+    for name in ("codec_test", "ans_ocl_test"):
+        src = os.path.join(ROOT, "tests", "cpp", name + ".cpp")
+        out = os.path.join(ROOT, "tests", "cpp", name)
+        outs.append(_atomic_build(out, lambda o, src=src: [
+            HIPCC, "--offload-arch=gfx950", "-O2", "-std=c++17", "-Wall", "-Wno-unused-command-line-argument",
+            "-I" + INCLUDE, src, "-L" + HERE, "-lgst_amd", "-Wl,-rpath," + HERE, "-o", o]))
+    return outs
 
 
 def build_oracle():

@@ -388,6 +388,8 @@ int DecodeStaging(const gst_header *hdrs, uint32_t n, hipStream_t s, const void 
 
 }  // namespace
 
+int gst::EnsureDecoderInit() { return EnsureInit(); }
+
 // ---------------------------------------------------------------------------
 // C ABI
 

@@ -157,4 +157,12 @@ inline uint32_t SinglePassGrid(uint32_t stage, uint32_t units, const StageConfig
 hipError_t LaunchStage(uint32_t stage, const KernelArgs &args, uint32_t grid,
                        const StageConfig &cfg, hipStream_t stream);
 
+// Stand-alone rANS groups (`groups` end offsets at the head of `data`, n = 1..32
+// interleaved streams each) against one packed table; out gets groups * n * 256 bytes.
+hipError_t LaunchAnsGroups(const uint32_t *table, const uint8_t *data, uint32_t data_len, uint32_t groups,
+                           uint32_t n, uint8_t *out, uint32_t *err, hipStream_t stream);
+
+// The decoder library's one-time device set-up (gst_initialize_decoder(0) on first use).
+int EnsureDecoderInit();
+
 }  // namespace gst

@@ -650,6 +650,65 @@ void AnsDecodeKernel(KernelArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Stand-alone interleaved rANS groups of 1..32 streams, in the layout the reference's
+// ANS decoder object packs (ans/ans_ocl.cpp:154-427) and its kernel reads
+// (ans/ans_decode.cl:25-95): `groups` u32 end offsets, then per group its 16-bit
+// renormalisation words in emission order followed by its n final states.  Same
+// step as the texture path: slot = x & 2047, x = (x >> 11) f + slot - cum, and lanes
+// with x < 2^15 take the next words in lane order.
+//
+// One wave per group; lanes >= n sit out of the ballot.  The table and the group's
+// last 8192 words (a group of 32 streams x 256 steps reads at most that many) are
+// copied to LDS first, so the serial chain reads only LDS; every global read is
+// range-checked through a buffer resource.  Stream j of group g leaves its 256
+// symbols at out[(g n + j) 256 ..], the last decoded first, 4 bytes per store.  A
+// malformed end offset, or a group that does not end in the encoder's initial state
+// 2^15, raises GST_DEVERR_STREAM_OFFSETS.
+constexpr uint32_t kGroupWindow = kLanesPerGroup * kSymsPerLane;  // 8192 words
+
+__global__ __launch_bounds__(64) void AnsGroupsKernel(const uint32_t *table, const uint8_t *data,
+                                                      uint32_t data_len, uint32_t n, uint8_t *out,
+                                                      uint32_t *err) {
+  __shared__ uint32_t tab[kTableSize];
+  __shared__ uint16_t words[kGroupWindow];
+  const uint32_t lane = threadIdx.x, g = blockIdx.x;
+  const GPtr<const uint32_t> t = Gbl<const uint32_t>(table);
+  for (uint32_t i = lane; i < kTableSize; i += 64) tab[i] = t[i];
+  const __amdgpu_buffer_rsrc_t rs = MakeRsrc(data, data_len);
+  const uint32_t end = __builtin_amdgcn_raw_buffer_load_b32(rs, static_cast<int>(4 * g), 0, 0);
+  const bool ok = (end & 3) == 0 && end <= data_len && end >= 4 * n;
+  const uint32_t p0 = ok ? (end - 4 * n) / 2 : 0u;
+  const uint32_t wbase = p0 > kGroupWindow ? p0 - kGroupWindow : 0u;
+  for (uint32_t i = lane; i < kGroupWindow; i += 64) {
+    const uint32_t w = wbase + i;
+    words[i] = w < p0 ? static_cast<uint16_t>(__builtin_amdgcn_raw_buffer_load_b16(rs, static_cast<int>(2 * w), 0, 0))
+                      : static_cast<uint16_t>(0);
+  }
+  const bool active = ok && lane < n;
+  uint32_t x = __builtin_amdgcn_raw_buffer_load_b32(rs, static_cast<int>(active ? end - 4 * n + 4 * lane : 0u), 0, 0);
+  x = active ? x : 0u;
+  __syncthreads();
+  uint32_t p = p0, acc = 0;
+  const GPtr<uint32_t> dst = Gbl<uint32_t>(reinterpret_cast<uint32_t *>(out + (static_cast<size_t>(g) * n + lane) *
+                                                                                 kSymsPerLane));
+  for (uint32_t i = 0; i < kSymsPerLane; ++i) {
+    const uint32_t e = tab[x & (kTableSize - 1)];
+    x = __umul24(x >> kTableLog, e & 0xFFFu) + __builtin_amdgcn_ubfe(e, 12, 11);
+    const bool need = active && x < kStateL;
+    const uint64_t m = __builtin_amdgcn_ballot_w64(need);
+    const uint32_t total = static_cast<uint32_t>(__popcll(m));
+    const uint32_t rank = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
+                                                    __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
+    if (need) x = (x << 16) | words[(p - total + rank - wbase) & (kGroupWindow - 1)];
+    p -= total;
+    acc = (acc << 8) | (e >> 24);
+    if ((i & 3) == 3 && active) dst[(kSymsPerLane - 1 - i) / 4] = acc;
+  }
+  const bool bad = (lane == 0 && !ok) || (active && x != kStateL);
+  if (__builtin_amdgcn_ballot_w64(bad) != 0 && lane == 0) atomicOr(err, kErrStreamOffsets);
+}
+
+// ---------------------------------------------------------------------------
 // Stage 3: palette indices (codec/decode_indices.cl:6-84 composed by
 // codec/decoder.cpp:304-393): idx[i] = sum_{j<=i} (diff[j] - 128), int32.
 //
@@ -1441,6 +1500,14 @@ hipError_t LaunchStage(uint32_t stage, const KernelArgs &args, uint32_t grid,
     default:
       return hipErrorInvalidValue;
   }
+}
+
+hipError_t LaunchAnsGroups(const uint32_t *table, const uint8_t *data, uint32_t data_len, uint32_t groups,
+                           uint32_t n, uint8_t *out, uint32_t *err, hipStream_t stream) {
+  if (groups == 0) return hipSuccess;
+  if (n == 0 || n > kLanesPerGroup) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(AnsGroupsKernel, dim3(groups), dim3(64), 0, stream, table, data, data_len, n, out, err);
+  return hipGetLastError();
 }
 
 }  // namespace gst

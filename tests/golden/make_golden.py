@@ -107,18 +107,57 @@ def main():
             groups["symbols_%d" % seed] = raw[516 + n:516 + n + 8192].copy()
         np.savez_compressed(os.path.join(HERE, "ref_ans_groups.npz"), **groups)
 
+    make_hist()
+
+
+def ref_hist(m, counts):
+    out = subprocess.run([DRIVER, "hist", str(m), *map(str, counts)], check=True,
+                         capture_output=True, text=True).stdout.split()
+    return [int(v) for v in out]
+
+
+def make_hist():
+    """ref_hist.json: GenerateHistogram outputs.  The ANS decoder tests' count vectors
+    (ans/ans_ocl_test.cpp:65,108,158,221,278), the histogram KAT, and seeded random
+    vectors: small and skewed counts, up to 256 symbols, and counts of 2^21 and more,
+    whose count * M wraps in the reference's 32-bit product."""
     hist = {}
     for key, counts in (("ocl_init", [3, 2, 1, 4, 3]),
                         ("ocl_rebuild", [80, 300, 2, 14, 1, 1, 1, 20]),
                         ("ocl_single", [12, 14, 17, 1, 1, 2, 372]),
+                        ("ocl_interleaved", [32, 186, 54, 8, 94, 35, 13, 21, 456, 789, 33, 215, 6, 54, 987,
+                                             54, 65, 13, 2, 1]),
+                        ("ocl_multiple", [65, 4, 6, 132, 135, 64, 879, 87, 456, 13, 2, 12, 33, 16, 546, 987,
+                                          98, 74, 65, 43, 21, 32, 1]),
                         ("hist_kat", [1, 2, 3, 4, 5, 6, 7, 8, 9, 10])):
         m = 256 if key == "hist_kat" else 2048
-        out = subprocess.run([DRIVER, "hist", str(m), *map(str, counts)], check=True,
-                             capture_output=True, text=True).stdout.split()
-        hist[key] = {"counts": counts, "M": m, "normalized": [int(v) for v in out]}
+        hist[key] = {"counts": counts, "M": m, "normalized": ref_hist(m, counts)}
+    rng = np.random.default_rng(2024)
+    cases = []
+    for i in range(48):
+        n = int(rng.integers(2, 257))
+        kind = i % 4
+        if kind == 0:
+            counts = rng.integers(0, 20, n)
+        elif kind == 1:
+            counts = rng.geometric(0.05, n) - 1
+        elif kind == 2:
+            counts = rng.integers(0, 1 << 24, n)
+        else:
+            counts = np.where(rng.random(n) < 0.3, rng.integers(1, 4, n), rng.integers(0, 5000, n))
+        counts = [int(c) for c in counts]
+        if not any(counts):
+            counts[0] = 1
+        m = 2048 if i % 6 else 4096
+        cases.append({"counts": counts, "M": m, "normalized": ref_hist(m, counts)})
+    hist["random"] = cases
     with open(os.path.join(HERE, "ref_hist.json"), "w") as f:
-        json.dump(hist, f, indent=1)
+        json.dump(hist, f, separators=(",", ":"))
+        f.write("\n")
 
 
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["hist"]:
+        make_hist()
+    else:
+        main()

@@ -21,7 +21,8 @@ def declared_functions(header):
     return sorted(set(re.findall(r"\b((?:gst|gste)_[a-z0-9_]+)\s*\(", text)))
 
 
-@pytest.mark.parametrize("header,lib", [("gst_decoder.h", "libgst_amd.so"), ("gst_encode.h", "libgst_encode.so")])
+@pytest.mark.parametrize("header,lib", [("gst_decoder.h", "libgst_amd.so"), ("gst_serve.h", "libgst_amd.so"),
+                                        ("gst_ans.h", "libgst_amd.so"), ("gst_encode.h", "libgst_encode.so")])
 def test_library_exports_every_declared_symbol(header, lib):
     names = declared_functions(header)
     assert len(names) >= 5
