@@ -304,6 +304,45 @@ def run_served(qps_list, duration_s, budget_gib=32, closed_loop=64, tuned=None):
     return out
 
 
+def run_ans_api(files, reps=3):
+    """The stand-alone rANS API (gst_ans_*, the OpenCLDecoder counterpart) over the
+    corpus' luma streams: per image, RebuildTable from its Y frequencies, then one Decode
+    of all its Y groups (32 interleaved streams each).  End to end through the blocking
+    API: host packing, upload, the groups kernel, read-back.  Every stream must decode back
+    to the encoder's initial state (the API reports a corrupt stream otherwise)."""
+    import gst_amd
+    prep = []
+    for f in files:
+        h = gst_amd.ParseHeader(f)
+        freqs = [int(v) for v in np.frombuffer(f[28:28 + 512], "<u2")]
+        y = f[28 + 2048:28 + 2048 + h.y_cmp_sz]
+        groups = 2 * h.num_blocks // 8192
+        ends = np.frombuffer(y[:4 * groups], "<u4")
+        blobs, states, lo = [], [], 4 * groups
+        for g in range(groups):
+            grp = y[lo:int(ends[g])]
+            blobs.append(grp[:-128])
+            states += [int(v) for v in np.frombuffer(grp[-128:], "<u4")]
+            lo = int(ends[g])
+        prep.append((freqs, blobs, states))
+    dec = gst_amd.AnsDecoder(prep[0][0], 32)
+    symbols = sum(len(p[2]) for p in prep) * 256
+    best = None
+    for _ in range(reps + 1):  # the first pass warms up
+        t0 = time.perf_counter()
+        for freqs, blobs, states in prep:
+            dec.rebuild_table(freqs)
+            dec.decode(states, blobs)
+        dt = time.perf_counter() - t0
+        best = dt if best is None else min(best, dt)
+    dec.close()
+    out = {"images": len(prep), "groups": symbols // 8192, "symbols": symbols, "seconds": round(best, 4),
+           "Msym_s": round(symbols / best / 1e6, 1),
+           "path": "per image: RebuildTable + one Decode of all luma groups, blocking, host to host"}
+    log("ans api: %s" % out)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -315,6 +354,7 @@ def main():
     ap.add_argument("--no-served", action="store_true", help="skip the served (Poisson arrivals) leg")
     ap.add_argument("--served-seconds", type=float, default=3.0)
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes (no autotune)")
+    ap.add_argument("--no-ans-api", action="store_true", help="skip the stand-alone rANS API leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device-to-device copy of 512 MiB (5x), the reference kernel for "
                          "scaling FETCH_SIZE / WRITE_SIZE counters to bytes")
@@ -420,6 +460,9 @@ def main():
             singles[name] = {"ms": round(ms_s, 4), "Mtexel_s": round(w * h / (ms_s * 1e-3) / 1e6, 1)}
         result["single_image"] = singles
         log("single images: %s" % singles)
+
+    if not args.no_ans_api:
+        result["ans_api"] = run_ans_api(files)
 
     if not args.no_served:
         result["served"] = run_served([200.0, 1000.0, 5000.0], args.served_seconds, tuned=tuned)
