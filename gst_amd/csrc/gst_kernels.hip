@@ -503,18 +503,34 @@ __device__ __forceinline__ const uint8_t *Uniform(const uint8_t *p) {
 __device__ __forceinline__ void LdsBarrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
 // The work of BuildTableKernel for one table, straight into the rANS kernel's LDS
-// arrays (256 <= THREADS): thread t < 256 brings frequency f[t]; a scan by the first
-// four waves, then every slot's symbol by the same binary lifting.  A table that does
-// not sum to 2048 becomes the identity table.  Returns whether the table is valid;
-// ends with a work-group barrier.
+// (256 <= THREADS): thread t < 256 brings frequency f[t].  The slot -> symbol map is a
+// count, not a search.  Wave w owns slots [w S, (w + 1) S), S = 2048 / waves, lane l
+// slot w S + 64 j + l of chunk j:
+//   A. the waves clear their slots and the marks-before counters; the first four waves
+//      scan f + (f != 0) << 24, giving every symbol its cum (low 24 bits: 256 u16 sum
+//      below 2^24) and its rank among the nonzero symbols (high bits);
+//   B. every nonzero symbol x writes fc[x] = f | cum << 12, sym_of_rank[rank] = x and a
+//      start mark at slot cum[x], and each wave of symbols adds to before[w] how many
+//      of its marks fall below slot w S (a ballot per boundary, one LDS add);
+//   C. per chunk, a ballot of the marks and mbcnt give the marks up to each slot, so
+//      slot s gets symbol sym_of_rank[before + marks in the wave's slots up to s - 1].
+// Every LDS access of A and C is one word per lane at consecutive addresses.  A table
+// that does not sum to 2048 becomes the identity table.  Returns whether the table is
+// valid; ends with a work-group barrier.
 template <uint32_t THREADS>
-__device__ __forceinline__ bool BuildTableInLds(uint32_t f, uint32_t *tab, uint32_t *cum, uint32_t *freq,
-                                                uint32_t *wsum) {
-  static_assert(THREADS >= 256, "one thread per frequency");
+__device__ __forceinline__ bool BuildTableInLds(uint32_t f, uint32_t *tab, uint32_t *fc, uint8_t *sym_of_rank,
+                                                uint32_t *wsum, uint32_t *before) {
+  constexpr uint32_t kWaves = THREADS / 64, kSlotsPerWave = kTableSize / kWaves, kChunks = kSlotsPerWave / 64;
+  static_assert(THREADS >= 256 && kWaves <= 8 && kChunks * 64 * kWaves == kTableSize, "launch shape");
   const uint32_t t = threadIdx.x, lane = t & 63, wave = Uniform(t >> 6);
+  uint32_t *const mine = tab + wave * kSlotsPerWave + lane;
+#pragma unroll
+  for (uint32_t j = 0; j < kChunks; ++j) mine[64 * j] = 0;
+  if (t < kWaves) before[t] = 0;
+  const uint32_t pk = f + (static_cast<uint32_t>(f != 0) << 24);
   uint32_t v = 0;
   if (t < 256) {
-    v = f;
+    v = pk;
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
       const uint32_t n = __shfl_up(v, off);
@@ -523,37 +539,40 @@ __device__ __forceinline__ bool BuildTableInLds(uint32_t f, uint32_t *tab, uint3
     if (lane == 63) wsum[wave] = v;
   }
   LdsBarrier();
-  if (t < 256) {
+  const bool valid = ((wsum[0] + wsum[1] + wsum[2] + wsum[3]) & 0xFFFFFFu) == kTableSize;
+  if (t < 256 && valid) {
     uint32_t add = 0;
     for (uint32_t w = 0; w < wave; ++w) add += wsum[w];
-    cum[t] = v + add - f;
-    freq[t] = f;
+    const uint32_t excl = add + v - pk, cum = excl & 0xFFFFFFu;
+    fc[t] = f | (cum << 12);
+    if (f != 0) {  // the table sums to 2048, so cum + f <= 2048: the mark is inside it
+      sym_of_rank[excl >> 24] = static_cast<uint8_t>(t);
+      tab[cum] = 1;
+    }
+#pragma unroll
+    for (uint32_t w = 1; w < kWaves; ++w) {
+      const uint32_t below = static_cast<uint32_t>(__popcll(__builtin_amdgcn_ballot_w64(f != 0 && cum < w * kSlotsPerWave)));
+      if (lane == 0 && below) atomicAdd(&before[w], below);
+    }
   }
-  const bool valid = wsum[0] + wsum[1] + wsum[2] + wsum[3] == kTableSize;
   LdsBarrier();
-  // The thread's slots t + i * THREADS search four at a time, level by level, so
-  // their LDS reads overlap (four at once is what the registers left free by the
-  // in-flight ring chunks allow without spilling).
-  constexpr uint32_t kPer = kTableSize / THREADS;
-  constexpr uint32_t kAtOnce = kPer < 4 ? kPer : 4;
-  const uint32_t last = cum[255];
-#pragma unroll 1
-  for (uint32_t i0 = 0; i0 < kPer; i0 += kAtOnce) {
-    uint32_t pos[kAtOnce];  // number of cum[0..254] <= slot
+  if (valid) {
+    uint32_t base = before[wave];
 #pragma unroll
-    for (uint32_t i = 0; i < kAtOnce; ++i) pos[i] = 0;
-#pragma unroll
-    for (uint32_t s = 128; s; s >>= 1) {
-#pragma unroll
-      for (uint32_t i = 0; i < kAtOnce; ++i)
-        if (cum[pos[i] + s - 1] <= t + (i0 + i) * THREADS) pos[i] += s;
+    for (uint32_t j = 0; j < kChunks; ++j) {
+      const uint32_t mark = mine[64 * j];
+      const uint64_t bal = __builtin_amdgcn_ballot_w64(mark != 0);
+      const uint32_t upto = base + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(bal >> 32),
+                                                             __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(bal), mark));
+      const uint32_t x = sym_of_rank[(upto - 1) & 255];
+      const uint32_t w = fc[x];
+      const uint32_t slot = wave * kSlotsPerWave + 64 * j + lane;
+      mine[64 * j] = (w & 0xFFFu) | ((slot - (w >> 12)) << 12) | (x << 24);
+      base += static_cast<uint32_t>(__popcll(bal));
     }
+  } else {
 #pragma unroll
-    for (uint32_t i = 0; i < kAtOnce; ++i) {
-      const uint32_t slot = t + (i0 + i) * THREADS;
-      const uint32_t x = last <= slot ? 255u : pos[i] - 1;
-      tab[slot] = valid ? freq[x] | ((slot - cum[x]) << 12) | (x << 24) : kTableSize | (slot << 12);
-    }
+    for (uint32_t j = 0; j < kChunks; ++j) mine[64 * j] = kTableSize | ((wave * kSlotsPerWave + 64 * j + lane) << 12);
   }
   LdsBarrier();
   return valid;
@@ -569,11 +588,12 @@ template <uint32_t WAVES, uint32_t GPW, bool kBuild>
 // With 4 or 8 waves LDS allows 7 waves per SIMD; hold the VGPRs to that (at most 72).
 __global__ __launch_bounds__(WAVES * 64, WAVES >= 4 ? 7 : 1)
 void AnsDecodeKernel(KernelArgs a) {
-  __shared__ uint32_t tab[kTableSize];
+  __shared__ __align__(16) uint32_t tab[kTableSize];
   __shared__ uint16_t ring[WAVES][2][kRingWords];
-  __shared__ uint32_t build_cum[kBuild ? 256 : 1];
-  __shared__ uint32_t build_freq[kBuild ? 256 : 1];
+  __shared__ uint32_t build_fc[kBuild ? 256 : 1];
+  __shared__ uint8_t build_sym[kBuild ? 256 : 1];
   __shared__ uint32_t build_wsum[4];
+  __shared__ uint32_t build_before[WAVES];
   __shared__ uint32_t touch_sink[64];
   constexpr uint32_t kGroupsPerUnit = WAVES * GPW;
   const uint32_t lane = threadIdx.x & 63, wave = Uniform(threadIdx.x >> 6);
@@ -624,7 +644,7 @@ void AnsDecodeKernel(KernelArgs a) {
     LdsBarrier();  // every wave is done with the previous unit's table and ring
     bool table_ok;
     if constexpr (kBuild) {
-      table_ok = BuildTableInLds<WAVES * 64>(f, tab, build_cum, build_freq, build_wsum);
+      table_ok = BuildTableInLds<WAVES * 64>(f, tab, build_fc, build_sym, build_wsum, build_before);
     } else {
       const uint32_t tix = Uniform(d.table_base + k);
       const GPtr<const uint32_t> src = Gbl<const uint32_t>(a.tables + static_cast<size_t>(tix) * kTableSize);
