@@ -491,6 +491,16 @@ __device__ __forceinline__ uint32_t Uniform(uint32_t v) {
   return static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(v)));
 }
 
+// Work-groups are dispatched to the 8 XCDs round robin (group b on XCD b % 8), each XCD
+// with its own L2.  This maps group b to a unit so that every XCD takes one contiguous
+// range of the grid's first pass: neighbouring units, which share cache lines and an
+// image's palette and sums, then meet in one L2.  A bijection on [0, n) for any n.
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t XcdContiguous(uint32_t b, uint32_t n) {
+  const uint32_t q = n / kXcds, r = n % kXcds, xcd = b % kXcds, idx = b / kXcds;
+  return (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + idx;
+}
+
 __device__ __forceinline__ const uint8_t *Uniform(const uint8_t *p) {
   const uint64_t v = reinterpret_cast<uint64_t>(p);
   const uint64_t lo = Uniform(static_cast<uint32_t>(v));
@@ -1248,7 +1258,7 @@ __global__ __launch_bounds__(kTileFusedWaves * 64, 5) void WaveletAssembleKernel
   const uint32_t lane = threadIdx.x & 63, half = lane >> 5, r = lane & 31;
   const uint32_t wave = Uniform(threadIdx.x >> 6);
   uint32_t *const L = lds[wave][half];
-  for (uint32_t u = blockIdx.x;; u += gridDim.x) {
+  for (uint32_t u = XcdContiguous(blockIdx.x, gridDim.x);; u += gridDim.x) {
     const Located loc = Locate(a, kStageAssemble, 2 * u, 1);
     if (!loc.valid) break;
     const DevDesc d = LoadDesc(a, loc.img);
