@@ -258,9 +258,13 @@ typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 
 // Lane l's 8 bytes of chunk c (words 4l .. 4l+3): one wave moves a whole chunk, so a
-// refill is never divergent.  Chunks below the stream start wrap to huge offsets and
-// read zeros.
-__device__ __forceinline__ u32x2 LoadChunk(__amdgpu_buffer_rsrc_t rs, uint32_t c, uint32_t l) {
+// refill is never divergent.  A chunk wholly below the group's first chunk `clo` (the
+// ring asks up to three chunks past the group's start near its end) reads nothing: its
+// offset is moved past the stream (c * 512 wraps to 0xFFFFFE00), as are chunks below
+// the stream start, which wrap by themselves.
+constexpr uint32_t kNoChunk = 0xFFFFFFFFu >> 9;
+__device__ __forceinline__ u32x2 LoadChunk(__amdgpu_buffer_rsrc_t rs, uint32_t c, uint32_t clo, uint32_t l) {
+  c = c >= clo ? c : kNoChunk;
   return __builtin_amdgcn_raw_buffer_load_b64(rs, static_cast<int>((c * kChunkWords + 4 * l) * 2), 0, 0);
 }
 
@@ -278,16 +282,17 @@ struct AnsRing {
 
 template <uint32_t GPW>
 __device__ __forceinline__ AnsRing RequestRing(__amdgpu_buffer_rsrc_t rs, uint32_t pA, uint32_t pB,
-                                               uint32_t lane) {
+                                               uint32_t loA, uint32_t loB, uint32_t lane) {
   AnsRing r;
   r.topA = pA ? (pA - 1) / kChunkWords : 0;
   r.topB = pB ? (pB - 1) / kChunkWords : 0;
-  r.rA0 = LoadChunk(rs, r.topA, lane);
-  r.rA1 = LoadChunk(rs, r.topA - 1, lane);
+  const uint32_t cloA = loA / kChunkWords, cloB = loB / kChunkWords;
+  r.rA0 = LoadChunk(rs, r.topA, cloA, lane);
+  r.rA1 = LoadChunk(rs, r.topA - 1, cloA, lane);
   r.rB0 = r.rB1 = u32x2{0u, 0u};
   if (GPW == 2) {
-    r.rB0 = LoadChunk(rs, r.topB, lane);
-    r.rB1 = LoadChunk(rs, r.topB - 1, lane);
+    r.rB0 = LoadChunk(rs, r.topB, cloB, lane);
+    r.rB1 = LoadChunk(rs, r.topB - 1, cloB, lane);
   }
   return r;
 }
@@ -343,12 +348,13 @@ __device__ __forceinline__ void DecodeHalfGroups(const uint32_t *tab, uint16_t *
   // refill's store is still vmcnt(0) (the number of refill loads between two crossings
   // is data-dependent and output stores are in flight), which TouchStreamTail makes
   // cheap.
-  u32x2 pAE = LoadChunk(rs, (topA & 1u) ? topA - 3 : topA - 2, lane);
-  u32x2 pAO = LoadChunk(rs, (topA & 1u) ? topA - 2 : topA - 3, lane);
+  const uint32_t cloA = loA / kChunkWords, cloB = loB / kChunkWords;
+  u32x2 pAE = LoadChunk(rs, (topA & 1u) ? topA - 3 : topA - 2, cloA, lane);
+  u32x2 pAO = LoadChunk(rs, (topA & 1u) ? topA - 2 : topA - 3, cloA, lane);
   u32x2 pBE = {0u, 0u}, pBO = {0u, 0u};
   if (GPW == 2) {
-    pBE = LoadChunk(rs, (topB & 1u) ? topB - 3 : topB - 2, lane);
-    pBO = LoadChunk(rs, (topB & 1u) ? topB - 2 : topB - 3, lane);
+    pBE = LoadChunk(rs, (topB & 1u) ? topB - 3 : topB - 2, cloB, lane);
+    pBO = LoadChunk(rs, (topB & 1u) ? topB - 2 : topB - 3, cloB, lane);
   }
   TouchStreamTail(rs, lane < 32 ? topA : topB, 2u * (lane < 32 ? loA : loB), j, sink);
   StoreChunk(ringA, topA, lane, ring.rA0);
@@ -421,22 +427,22 @@ __device__ __forceinline__ void DecodeHalfGroups(const uint32_t *tab, uint16_t *
       if (crossA && oddA) {
         StoreChunk(ringA, topA - 2, lane, pAO);
         asm volatile("" ::: "memory");
-        pAO = LoadChunk(rs, topA - 4, lane);
+        pAO = LoadChunk(rs, topA - 4, cloA, lane);
       }
       if (crossA && !oddA) {
         StoreChunk(ringA, topA - 2, lane, pAE);
         asm volatile("" ::: "memory");
-        pAE = LoadChunk(rs, topA - 4, lane);
+        pAE = LoadChunk(rs, topA - 4, cloA, lane);
       }
       if (GPW == 2 && crossB && oddB) {
         StoreChunk(ringB, topB - 2, lane, pBO);
         asm volatile("" ::: "memory");
-        pBO = LoadChunk(rs, topB - 4, lane);
+        pBO = LoadChunk(rs, topB - 4, cloB, lane);
       }
       if (GPW == 2 && crossB && !oddB) {
         StoreChunk(ringB, topB - 2, lane, pBE);
         asm volatile("" ::: "memory");
-        pBE = LoadChunk(rs, topB - 4, lane);
+        pBE = LoadChunk(rs, topB - 4, cloB, lane);
       }
       topA -= crossA ? 1u : 0u;
       topB -= crossB ? 1u : 0u;
@@ -649,7 +655,7 @@ void AnsDecodeKernel(KernelArgs a) {
     const uint32_t loA = __builtin_amdgcn_readlane(plo, 0);
     const uint32_t pB = GPW == 2 ? __builtin_amdgcn_readlane(p, 32) : 0u;
     const uint32_t loB = GPW == 2 ? __builtin_amdgcn_readlane(plo, 32) : 0u;
-    const AnsRing chunks = RequestRing<GPW>(rs, pA, pB, lane);
+    const AnsRing chunks = RequestRing<GPW>(rs, pA, pB, loA, loB, lane);
 
     LdsBarrier();  // every wave is done with the previous unit's table and ring
     bool table_ok;
