@@ -214,3 +214,51 @@ def test_full_plan_of_many_small_images(device):
         assert b.error_bits(device.stream) == 0
     finally:
         b.close()
+
+
+@pytest.mark.parametrize("ans_grid,asm_grid", [(9, 13), (3, 1), (1568, 7)])
+def test_odd_persistent_grids(device, ans_grid, asm_grid):
+    """Grids that are not multiples of the 8 XCDs: the fused kernel's XCD-contiguous unit
+    order must still visit every unit exactly once, pass after pass."""
+    files, dxts = encode.make_textures(MIXED[:5])
+    blocks, pal = _capacity(files)
+    b = gst_amd.Batch(len(files), blocks, pal, grids={"ans_decode": ans_grid, "assemble": asm_grid})
+    try:
+        _, _, d_outs = _bind(device, b, files)
+        b.decode(device.stream)
+        for d, out in zip(dxts, d_outs):
+            assert device.download(out) == d
+        assert b.error_bits(device.stream) == 0
+    finally:
+        b.close()
+
+
+def test_two_plans_in_flight(device):
+    """Two plans bound to the same inputs, each with its own outputs and stream, used
+    alternately so that their decodes overlap on the device."""
+    torch = device.torch
+    files, dxts = encode.make_textures(MIXED[:6])
+    blocks, pal = _capacity(files)
+    hdrs = [gst_amd.ParseHeader(f) for f in files]
+    d_files = [device.upload(f) for f in files]
+    streams = [torch.cuda.Stream(device=device.dev), torch.cuda.Stream(device=device.dev)]
+    plans, outs = [], []
+    try:
+        for st in streams:
+            p = gst_amd.Batch(len(files), blocks, pal)
+            o = [device.empty(h.dxt_bytes) for h in hdrs]
+            p.bind(hdrs, [t.data_ptr() for t in d_files], [t.data_ptr() for t in o], st.cuda_stream)
+            plans.append(p)
+            outs.append(o)
+        torch.cuda.synchronize()
+        for k in range(8):
+            plans[k % 2].decode(streams[k % 2].cuda_stream)
+        torch.cuda.synchronize()
+        for o in outs:
+            for d, out in zip(dxts, o):
+                assert device.download(out) == d
+        for p, st in zip(plans, streams):
+            assert p.error_bits(st.cuda_stream) == 0
+    finally:
+        for p in plans:
+            p.close()
