@@ -58,8 +58,11 @@ def build_decoder():
 def build_encoder():
     out = os.path.join(HERE, "libgst_encode.so")
     cxx = shutil.which("g++") or "g++"
+    # -ffp-contract=off: gste_compress_dxt repeats the reference encoder's float
+    # arithmetic operation by operation, with no fused multiply-adds.
     return _atomic_build(out, lambda o: [cxx, "-std=c++17", "-O2", "-Wall", "-Wextra", "-fPIC", "-shared",
-                                         "-pthread", "-I" + INCLUDE, os.path.join(CSRC, "gst_encode.cpp"),
+                                         "-pthread", "-ffp-contract=off", "-I" + INCLUDE, "-I" + CSRC,
+                                         os.path.join(CSRC, "gst_encode.cpp"), os.path.join(CSRC, "gst_gentc.cpp"),
                                          "-o", o])
 
 

@@ -1,7 +1,8 @@
-"""Seeded synthetic textures and the CPU GST encoder (libgst_encode.so, include/gst_encode.h).
+"""The GST encoder and seeded synthetic textures (libgst_encode.so, include/gst_encode.h).
 
-Input tooling for tests and benchmarks: makes .gtc files in the reference's on-disk
-layout from deterministic textures, on any host.
+`compress_dxt` is the reference encoder gentenc (GenTC::CompressDXT) byte for byte.
+The rest is input tooling for tests and benchmarks: a fast encoder that makes .gtc
+files in the reference's on-disk layout from deterministic textures, on any host.
 """
 import ctypes
 import os
@@ -39,11 +40,63 @@ def lib():
         h.gste_make_texture.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                         ctypes.c_uint32, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p,
                                         ctypes.c_void_p]
+        h.gste_compress_dxt.restype = ctypes.c_int
+        h.gste_compress_dxt.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p,
+                                        ctypes.c_void_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t),
+                                        ctypes.c_void_p]
         h.gste_make_textures.argtypes = [ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                          ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p,
                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
         _lib = h
     return _lib
+
+
+GSTE_OK, GSTE_ERR_ARGUMENT, GSTE_ERR_CAPACITY, GSTE_ERR_RANGE, GSTE_ERR_UNCODABLE = 0, -1, -2, -3, -4
+
+
+class EncodeError(ValueError):
+    """gste_compress_dxt failed; `code` is the GSTE_* status."""
+
+    def __init__(self, code, what):
+        super().__init__("%s (status %d)" % (what, code))
+        self.code = code
+
+
+def compress_dxt(rgb, width=None, height=None, dxt=None):
+    """GenTC::CompressDXT (codec/encoder.cpp:176-187): the .gtc file gentenc writes for
+    an RGB8 image, and the DXT1 blocks it decodes to.  `rgb`: (height, width, 3) uint8
+    array, or flat bytes with `width` and `height`.  `dxt`: optional starting blocks
+    (width*height/2 bytes, DXTImage(w, h, rgb, dxt)); None compresses them with stb_dxt
+    as gentenc does.  Returns (gtc bytes, dxt bytes)."""
+    a = np.ascontiguousarray(np.frombuffer(rgb, np.uint8) if isinstance(rgb, (bytes, bytearray)) else rgb,
+                             dtype=np.uint8)
+    if a.ndim == 3:
+        height, width = a.shape[0], a.shape[1]
+        if a.shape[2] != 3:
+            raise ValueError("rgb must have 3 channels")
+    if width is None or height is None or a.size != width * height * 3:
+        raise ValueError("rgb size does not match width x height x 3")
+    blocks = None
+    if dxt is not None:
+        blocks = np.ascontiguousarray(np.frombuffer(bytes(dxt), np.uint8))
+        if blocks.size != width * height // 2:
+            raise ValueError("dxt must hold width*height/2 bytes")
+    need = ctypes.c_size_t()
+    cap = lib().gste_max_encoded_size(width, height)
+    out = np.empty(cap, np.uint8)
+    blk = np.empty(width * height // 2, np.uint8)
+    rc = lib().gste_compress_dxt(a.ctypes.data, width, height, None if blocks is None else blocks.ctypes.data,
+                                 out.ctypes.data, cap, ctypes.byref(need), blk.ctypes.data)
+    if rc == GSTE_ERR_CAPACITY:  # not expected: the bound covers every legal input
+        out = np.empty(need.value, np.uint8)
+        rc = lib().gste_compress_dxt(a.ctypes.data, width, height,
+                                     None if blocks is None else blocks.ctypes.data,
+                                     out.ctypes.data, need.value, ctypes.byref(need), blk.ctypes.data)
+    if rc != GSTE_OK:
+        raise EncodeError(rc, {GSTE_ERR_ARGUMENT: "size gentenc cannot code",
+                               GSTE_ERR_RANGE: "palette-index step outside a byte",
+                               GSTE_ERR_UNCODABLE: "a stream of zero bytes only"}.get(rc, "failed"))
+    return out[:need.value].tobytes(), blk.tobytes()
 
 
 def generate_rgb(seed, width, height, noise_milli=DEFAULT_NOISE_MILLI):

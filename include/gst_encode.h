@@ -1,12 +1,13 @@
 /*
- * gst_encode.h -- seeded synthetic textures and a CPU GST encoder (libgst_encode.so).
+ * gst_encode.h -- the GST encoder and seeded synthetic textures (libgst_encode.so).
  *
- * Input tooling for tests and benchmarks: it produces .gtc files in the reference's
- * on-disk layout (GammaUNC/GST codec/encoder.cpp:122-144, codec/entropy.cpp:174-265)
- * from seeded synthetic RGB8 textures.  It is not byte-identical to the reference
- * encoder `gentenc` (different DXT1 compressor and palette-reuse search); every file
- * it writes decodes with the reference decoder semantics, and the DXT1 blocks it
- * reports are what a correct decoder must return.
+ * gste_compress_dxt is the reference encoder `gentenc` (GenTC::CompressDXT) byte for
+ * byte: same DXT1 blocks, same .gtc file.  The other entry points are input tooling
+ * for tests and benchmarks: a fast encoder (own DXT1 compressor and palette-reuse
+ * search, not byte-identical to gentenc) over seeded synthetic RGB8 textures.  Every
+ * file either writes has the reference's on-disk layout (GammaUNC/GST
+ * codec/encoder.cpp:122-144, codec/entropy.cpp:174-265), and the DXT1 blocks they
+ * report are what a correct decoder must return.
  */
 #ifndef GST_ENCODE_H_
 #define GST_ENCODE_H_
@@ -25,6 +26,30 @@ typedef struct gste_stats {
                                     decodes to different endpoints for that tile) */
   uint32_t reserved;
 } gste_stats;
+
+/* Status codes of gste_compress_dxt. */
+#define GSTE_OK 0
+#define GSTE_ERR_ARGUMENT (-1) /* NULL rgb/out_size, or a size gentenc cannot code: width and
+                                  height multiples of 128 and width*height of 131072 */
+#define GSTE_ERR_CAPACITY (-2) /* out is NULL or out_cap is short; *out_size holds the size */
+#define GSTE_ERR_RANGE (-3)    /* a palette-index step does not fit a byte (gentenc asserts;
+                                  its 127-entry lookback does not produce one) */
+#define GSTE_ERR_UNCODABLE (-4) /* a stream holds zero bytes only, which gentenc's rANS
+                                  coder refuses on an assertion (ans/ans_utils.h:42-47) */
+
+/* GenTC::CompressDXT(width, height, rgb, dxt) and GenTC::CompressDXT(DXTImage(width,
+ * height, rgb)) (codec/encoder.cpp:176-187, codec/dxt_image.cpp:444-468): the .gtc
+ * file gentenc writes for an RGB8 image (`rgb`, width*height*3 bytes, rows top down).
+ * `dxt_in` (width*height/2 bytes) gives the DXT1 blocks to start from; when NULL the
+ * blocks are compressed as gentenc does, with stb_dxt v1.04 in its high-quality mode
+ * (codec/dxt_image.cpp:362-385).  The blocks then go through gentenc's palette reuse
+ * (codec/dxt_image.cpp:512-624), and `dxt_out` (width*height/2 bytes, may be NULL)
+ * receives them: the image the file decodes to.  *out_size receives the file size
+ * (also when out_cap is short).  Returns a GSTE_* status.  As in gentenc, a wavelet
+ * coefficient outside [-128, 127] wraps, and such a file decodes to other endpoints
+ * than `dxt_out`.  Single-threaded, no state between calls: concurrent calls are safe. */
+int gste_compress_dxt(const uint8_t *rgb, uint32_t width, uint32_t height, const uint8_t *dxt_in,
+                      uint8_t *out, size_t out_cap, size_t *out_size, uint8_t *dxt_out);
 
 /* Seeded synthetic RGB8 texture: two linear gradients per channel, 4-octave value
  * noise of amplitude `noise_milli`/1000 of the 0..255 range, and 8 hard-edged

@@ -13,6 +13,11 @@
 //       PhysicalBlocks(), the expected decoder output (codec/test/codec_test.cpp:44-47).
 //   encode_png <image.png> <out.gtc> <out.dxt>
 //       Same, image loaded by DXTImage(orig_fn, NULL) (codec/dxt_image.cpp:387-442).
+//   encode_rgb_dxt <rgb8.raw> <w> <h> <in.dxt> <out.gtc> <out.dxt>
+//       GenTC::DXTImage(w, h, rgb, dxt) (given blocks + Reencode,
+//       codec/dxt_image.cpp:454-468) then GenTC::CompressDXT.
+//   png_rgb <image.png> <out.rgb>
+//       The RGB8 pixels stbi_load(fn, .., 3) gives DXTImage(orig_fn, NULL).
 //   stages <in.gtc> <out.sym> <out.planes>
 //       Per-group ans::DecodeInterleaved (ans/decode.cpp:240-292) of every stream
 //       into the reference symbol layout, then InverseWavelet2D
@@ -42,6 +47,9 @@
 #include "encoder.h"
 #include "histogram.h"
 #include "wavelet.h"
+
+// Declarations only; the implementation is compiled into codec/dxt_image.cpp.
+#include "stb_image.h"
 
 static std::vector<uint8_t> ReadFile(const char *fn) {
   std::ifstream is(fn, std::ios::binary);
@@ -98,6 +106,33 @@ static int CmdEncodePNG(int argc, char **argv) {
   GenTC::DXTImage img(argv[2], NULL);
   delete q;
   WriteEncoded(img, argv[3], argv[4]);
+  return 0;
+}
+
+static int CmdEncodeRGBDXT(int argc, char **argv) {
+  if (argc != 8) return 1;
+  std::vector<uint8_t> rgb = ReadFile(argv[2]);
+  const int w = atoi(argv[3]), h = atoi(argv[4]);
+  std::vector<uint8_t> dxt = ReadFile(argv[5]);
+  if (rgb.size() != static_cast<size_t>(w) * h * 3 || dxt.size() != static_cast<size_t>(w) * h / 2) {
+    fprintf(stderr, "rgb or dxt size mismatch\n");
+    return 2;
+  }
+  QuietCout *q = new QuietCout;
+  GenTC::DXTImage img(w, h, rgb, dxt);
+  delete q;
+  WriteEncoded(img, argv[6], argv[7]);
+  return 0;
+}
+
+static int CmdPngRGB(int argc, char **argv) {
+  if (argc != 4) return 1;
+  int w = 0, h = 0;
+  stbi_uc *data = stbi_load(argv[2], &w, &h, NULL, 3);
+  if (!data) return 2;
+  WriteFile(argv[3], data, static_cast<size_t>(w) * h * 3);
+  stbi_image_free(data);
+  printf("%d %d\n", w, h);
   return 0;
 }
 
@@ -199,13 +234,15 @@ static int CmdAnsGroup(int argc, char **argv) {
 
 int main(int argc, char **argv) {
   if (argc < 2) {
-    fprintf(stderr, "usage: ref_driver encode_rgb|encode_png|stages|hist|ans_group ...\n");
+    fprintf(stderr, "usage: ref_driver encode_rgb|encode_png|encode_rgb_dxt|png_rgb|stages|hist|ans_group ...\n");
     return 1;
   }
   const std::string cmd = argv[1];
   int rc = 1;
   if (cmd == "encode_rgb") rc = CmdEncodeRGB(argc, argv);
   else if (cmd == "encode_png") rc = CmdEncodePNG(argc, argv);
+  else if (cmd == "encode_rgb_dxt") rc = CmdEncodeRGBDXT(argc, argv);
+  else if (cmd == "png_rgb") rc = CmdPngRGB(argc, argv);
   else if (cmd == "stages") rc = CmdStages(argc, argv);
   else if (cmd == "hist") rc = CmdHist(argc, argv);
   else if (cmd == "ans_group") rc = CmdAnsGroup(argc, argv);

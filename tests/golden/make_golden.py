@@ -15,6 +15,9 @@ its outputs as data fixtures:
   ref_small.npz     a 512x256 texture (smallest legal size, one index group).
   ref_wide.npz      a 1024x512 texture (three index-scan stages).
                     (These two keep only `gtc` and `dxt`.)
+  ref_given.npz     a 512x256 texture encoded from given blocks, DXTImage(w, h, rgb, dxt)
+                    with this repository's fast DXT1 compressor's blocks.
+  (ref_test1.npz also keeps `rgb`, the pixels stbi_load gives the encoder.)
   ref_ans_groups.npz  single-group rANS known answers from ans::EncodeInterleaved.
   ref_hist.json     ans::GenerateHistogram outputs used by the table known answers
                     of ans/ans_ocl_test.cpp:64-155.
@@ -47,15 +50,21 @@ def read(path):
 
 
 def encode_case(name, rgb_path=None, png=None, w=None, h=None, meta=None, tmp=None,
-                stages=True):
+                stages=True, dxt_in=None):
     gtc, dxt = os.path.join(tmp, name + ".gtc"), os.path.join(tmp, name + ".dxt")
     sym, planes = os.path.join(tmp, name + ".sym"), os.path.join(tmp, name + ".planes")
+    extra = {}
     if png:
         run("encode_png", png, gtc, dxt)
+        rgb = os.path.join(tmp, name + ".rgb")
+        run("png_rgb", png, rgb)
+        extra["rgb"] = np.frombuffer(read(rgb), np.uint8)  # the encoder's input pixels
+    elif dxt_in:
+        run("encode_rgb_dxt", rgb_path, w, h, dxt_in, gtc, dxt)
     else:
         run("encode_rgb", rgb_path, w, h, gtc, dxt)
     arrays = dict(gtc=np.frombuffer(read(gtc), np.uint8), dxt=np.frombuffer(read(dxt), np.uint8),
-                  meta=np.frombuffer(json.dumps(meta or {}).encode(), np.uint8))
+                  meta=np.frombuffer(json.dumps(meta or {}).encode(), np.uint8), **extra)
     if stages:
         run("stages", gtc, sym, planes)
         arrays["symbols"] = np.frombuffer(read(sym), np.uint8)
@@ -67,6 +76,16 @@ def synthetic_rgb(seed, w, h, noise_milli, path):
     lib = ctypes.CDLL(os.path.join(ROOT, "gst_amd", "libgst_encode.so"))
     buf = ctypes.create_string_buffer(w * h * 3)
     lib.gste_generate_rgb(ctypes.c_uint64(seed), w, h, noise_milli, buf)
+    with open(path, "wb") as f:
+        f.write(buf.raw)
+    return buf
+
+
+def fast_dxt1(rgb, w, h, path):
+    """Blocks of this repository's fast DXT1 compressor (gste_compress_dxt1)."""
+    lib = ctypes.CDLL(os.path.join(ROOT, "gst_amd", "libgst_encode.so"))
+    buf = ctypes.create_string_buffer(w * h // 2)
+    lib.gste_compress_dxt1(rgb, w, h, buf)
     with open(path, "wb") as f:
         f.write(buf.raw)
 
@@ -85,6 +104,14 @@ def main():
             encode_case(name, rgb_path=rgb, w=w, h=h, tmp=tmp, stages=(name == "ref_c0"),
                         meta={"generator": "gste_generate_rgb", "seed": seed, "width": w,
                               "height": h, "noise_milli": 300})
+        # Given blocks: DXTImage(w, h, rgb, dxt) with the fast compressor's blocks.
+        rgb = os.path.join(tmp, "given.rgb")
+        pixels = synthetic_rgb(21, 512, 256, 300, rgb)
+        blocks = os.path.join(tmp, "given.dxt_in")
+        fast_dxt1(pixels, 512, 256, blocks)
+        encode_case("ref_given", rgb_path=rgb, w=512, h=256, tmp=tmp, stages=False, dxt_in=blocks,
+                    meta={"generator": "gste_generate_rgb", "seed": 21, "width": 512, "height": 256,
+                          "noise_milli": 300, "blocks": "gste_compress_dxt1"})
 
         # Single-group rANS known answers.  The first count vector is the one of
         # ans/ans_ocl_test.cpp DecodeSingleStream; the others cover a 256-symbol
