@@ -590,4 +590,39 @@ void gste_make_textures(uint32_t count, const uint64_t *seeds, const uint32_t *w
   for (auto &t : th) t.join();
 }
 
+void gste_make_textures_gentenc(uint32_t count, const uint64_t *seeds, const uint32_t *widths,
+                                const uint32_t *heights, uint32_t noise_milli, uint8_t *out,
+                                const uint64_t *out_offsets, uint8_t *dxt, const uint64_t *dxt_offsets,
+                                uint64_t *sizes, int nthreads) {
+  // Largest first, so that the big textures do not start last.
+  std::vector<uint32_t> order(count);
+  for (uint32_t i = 0; i < count; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+    return static_cast<uint64_t>(widths[a]) * heights[a] > static_cast<uint64_t>(widths[b]) * heights[b];
+  });
+  std::atomic<uint32_t> next(0);
+  auto work = [&]() {
+    std::vector<uint8_t> rgb;
+    for (;;) {
+      const uint32_t k = next.fetch_add(1);
+      if (k >= count) break;
+      const uint32_t i = order[k];
+      sizes[i] = 0;
+      if (widths[i] == 0 || heights[i] == 0 || widths[i] % 128 || heights[i] % 128) continue;
+      rgb.resize(3 * static_cast<size_t>(widths[i]) * heights[i]);
+      gste_generate_rgb(seeds[i], widths[i], heights[i], noise_milli, rgb.data());
+      size_t n = 0;
+      if (gste_compress_dxt(rgb.data(), widths[i], heights[i], nullptr, out + out_offsets[i],
+                            gste_max_encoded_size(widths[i], heights[i]), &n,
+                            dxt ? dxt + dxt_offsets[i] : nullptr) == GSTE_OK)
+        sizes[i] = n;
+    }
+  };
+  const int n = std::max(1, std::min(nthreads, static_cast<int>(count)));
+  std::vector<std::thread> th;
+  for (int t = 1; t < n; ++t) th.emplace_back(work);
+  work();
+  for (auto &t : th) t.join();
+}
+
 }  // extern "C"

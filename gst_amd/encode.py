@@ -44,6 +44,10 @@ def lib():
         h.gste_compress_dxt.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p,
                                         ctypes.c_void_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t),
                                         ctypes.c_void_p]
+        h.gste_make_textures_gentenc.argtypes = [ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p,
+                                                 ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p,
+                                                 ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                                 ctypes.c_void_p, ctypes.c_int]
         h.gste_make_textures.argtypes = [ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                          ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p,
                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
@@ -119,8 +123,12 @@ def make_texture(seed, width, height, noise_milli=DEFAULT_NOISE_MILLI, reuse=DEF
 
 
 def make_textures(specs, noise_milli=DEFAULT_NOISE_MILLI, reuse=DEFAULT_REUSE, nthreads=None,
-                  want_dxt=True):
-    """specs: list of (seed, width, height).  Returns (files, dxts) lists of bytes."""
+                  want_dxt=True, encoder="fast"):
+    """specs: list of (seed, width, height).  Returns (files, dxts) lists of bytes.
+    encoder "gentenc": every file byte for byte the reference encoder's (stb_dxt blocks,
+    gentenc's palette reuse; `reuse` is ignored); "fast": the corpus encoder."""
+    if encoder not in ("fast", "gentenc"):
+        raise ValueError("encoder must be 'fast' or 'gentenc'")
     n = len(specs)
     seeds = np.array([s[0] for s in specs], np.uint64)
     ws = np.array([s[1] for s in specs], np.uint32)
@@ -133,9 +141,15 @@ def make_textures(specs, noise_milli=DEFAULT_NOISE_MILLI, reuse=DEFAULT_REUSE, n
     dxt = np.empty(int(dx.sum()) if want_dxt else 1, np.uint8)
     sizes = np.zeros(n, np.uint64)
     threads = nthreads or min(16, os.cpu_count() or 1)
-    lib().gste_make_textures(n, seeds.ctypes.data, ws.ctypes.data, hs.ctypes.data, noise_milli, reuse,
-                             out.ctypes.data, out_off.ctypes.data, dxt.ctypes.data if want_dxt else None,
-                             dxt_off.ctypes.data, sizes.ctypes.data, threads)
+    if encoder == "gentenc":
+        lib().gste_make_textures_gentenc(n, seeds.ctypes.data, ws.ctypes.data, hs.ctypes.data, noise_milli,
+                                         out.ctypes.data, out_off.ctypes.data,
+                                         dxt.ctypes.data if want_dxt else None, dxt_off.ctypes.data,
+                                         sizes.ctypes.data, threads)
+    else:
+        lib().gste_make_textures(n, seeds.ctypes.data, ws.ctypes.data, hs.ctypes.data, noise_milli, reuse,
+                                 out.ctypes.data, out_off.ctypes.data, dxt.ctypes.data if want_dxt else None,
+                                 dxt_off.ctypes.data, sizes.ctypes.data, threads)
     if not all(sizes):
         raise ValueError("encoding failed for some textures")
     files = [out[int(o):int(o) + int(s)].tobytes() for o, s in zip(out_off, sizes)]

@@ -86,6 +86,17 @@ void gste_make_textures(uint32_t count, const uint64_t *seeds, const uint32_t *w
                         uint8_t *out, const uint64_t *out_offsets, uint8_t *dxt,
                         const uint64_t *dxt_offsets, uint64_t *sizes, int nthreads);
 
+/* The same textures encoded as gentenc does: texture i is gste_generate_rgb(seeds[i],
+ * widths[i], heights[i], noise_milli) through gste_compress_dxt (stb_dxt blocks, then
+ * gentenc's palette reuse), so each file is byte for byte the reference encoder's.
+ * Textures are taken largest first by `nthreads` threads; out + out_offsets[i] holds
+ * gste_max_encoded_size bytes; dxt (may be NULL) receives the blocks the file decodes
+ * to.  sizes[i] = file size, or 0 when gste_compress_dxt refused the texture. */
+void gste_make_textures_gentenc(uint32_t count, const uint64_t *seeds, const uint32_t *widths,
+                                const uint32_t *heights, uint32_t noise_milli, uint8_t *out,
+                                const uint64_t *out_offsets, uint8_t *dxt, const uint64_t *dxt_offsets,
+                                uint64_t *sizes, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif
