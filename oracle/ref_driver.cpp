@@ -18,6 +18,9 @@
 //       codec/dxt_image.cpp:454-468) then GenTC::CompressDXT.
 //   png_rgb <image.png> <out.rgb>
 //       The RGB8 pixels stbi_load(fn, .., 3) gives DXTImage(orig_fn, NULL).
+//   decompress_rgb <in.dxt> <w> <h> <out.rgb>
+//       GenTC::DXTImage(w, h, dxt) (codec/dxt_image.cpp:470-480, no re-encoding), then
+//       DecompressedImage() (codec/dxt_image.cpp:665-687) with the alpha dropped.
 //   stages <in.gtc> <out.sym> <out.planes>
 //       Per-group ans::DecodeInterleaved (ans/decode.cpp:240-292) of every stream
 //       into the reference symbol layout, then InverseWavelet2D
@@ -136,6 +139,29 @@ static int CmdPngRGB(int argc, char **argv) {
   return 0;
 }
 
+static int CmdDecompressRGB(int argc, char **argv) {
+  if (argc != 6) return 1;
+  std::vector<uint8_t> dxt = ReadFile(argv[2]);
+  const int w = atoi(argv[3]), h = atoi(argv[4]);
+  if (dxt.size() != static_cast<size_t>(w) * h / 2) {
+    fprintf(stderr, "dxt size mismatch\n");
+    return 2;
+  }
+  GenTC::DXTImage img(w, h, dxt);
+  std::unique_ptr<GenTC::RGBAImage> rgba = img.DecompressedImage();
+  std::vector<uint8_t> rgb;
+  rgb.reserve(static_cast<size_t>(w) * h * 3);
+  for (int y = 0; y < h; ++y)
+    for (int x = 0; x < w; ++x) {
+      const GenTC::RGBA p = rgba->GetAt(x, y);
+      rgb.push_back(std::get<0>(p));
+      rgb.push_back(std::get<1>(p));
+      rgb.push_back(std::get<2>(p));
+    }
+  WriteFile(argv[5], rgb.data(), rgb.size());
+  return 0;
+}
+
 static uint32_t Rd32(const uint8_t *p) {
   uint32_t v;
   memcpy(&v, p, 4);
@@ -234,7 +260,7 @@ static int CmdAnsGroup(int argc, char **argv) {
 
 int main(int argc, char **argv) {
   if (argc < 2) {
-    fprintf(stderr, "usage: ref_driver encode_rgb|encode_png|encode_rgb_dxt|png_rgb|stages|hist|ans_group ...\n");
+    fprintf(stderr, "usage: ref_driver encode_rgb|encode_png|encode_rgb_dxt|png_rgb|decompress_rgb|stages|hist|ans_group ...\n");
     return 1;
   }
   const std::string cmd = argv[1];
@@ -243,6 +269,7 @@ int main(int argc, char **argv) {
   else if (cmd == "encode_png") rc = CmdEncodePNG(argc, argv);
   else if (cmd == "encode_rgb_dxt") rc = CmdEncodeRGBDXT(argc, argv);
   else if (cmd == "png_rgb") rc = CmdPngRGB(argc, argv);
+  else if (cmd == "decompress_rgb") rc = CmdDecompressRGB(argc, argv);
   else if (cmd == "stages") rc = CmdStages(argc, argv);
   else if (cmd == "hist") rc = CmdHist(argc, argv);
   else if (cmd == "ans_group") rc = CmdAnsGroup(argc, argv);

@@ -21,6 +21,7 @@ its outputs as data fixtures:
   ref_ans_groups.npz  single-group rANS known answers from ans::EncodeInterleaved.
   ref_hist.json     ans::GenerateHistogram outputs used by the table known answers
                     of ans/ans_ocl_test.cpp:64-155.
+  ref_rgb.npz       RGB8 known answers of the LoadRGB path (see make_rgb).
 
 The synthetic RGB comes from this repository's generator (libgst_encode.so), so
 the same pixels can be regenerated anywhere; the encoded bytes are the reference's.
@@ -183,8 +184,69 @@ def make_hist():
         f.write("\n")
 
 
+def four_colour_blocks(dxt):
+    """Blocks that the reference's CPU decompression (DXTImage::DecompressedImage, which
+    follows the DXT1 endpoint-order rule) turns into what assemble_rgb
+    (codec/assemble.cl:83-130, always four colours) computes for `dxt`: where ep1 < ep2
+    the endpoints swap and every index flips its low bit (then ep1 > ep2 and each pixel
+    keeps its colour: 0<->1, 2<->3); where ep1 == ep2 index 3 becomes 0 (both are ep1
+    among four colours, but black in the three-colour rule)."""
+    blk = np.frombuffer(bytes(dxt), "<u2").reshape(-1, 4).copy()
+    e0, e1 = blk[:, 0].copy(), blk[:, 1].copy()
+    word = blk[:, 2].astype(np.uint32) | (blk[:, 3].astype(np.uint32) << 16)
+    lt, eq = e0 < e1, e0 == e1
+    blk[lt, 0], blk[lt, 1] = e1[lt], e0[lt]
+    word[lt] ^= np.uint32(0x55555555)
+    weq = word[eq]
+    for i in range(16):
+        weq[((weq >> np.uint32(2 * i)) & 3) == 3] &= ~np.uint32(3 << (2 * i))
+    word[eq] = weq
+    blk[:, 2] = word & 0xFFFF
+    blk[:, 3] = word >> 16
+    return blk.tobytes()
+
+
+def reference_rgb(dxt, w, h, tmp):
+    src, out = os.path.join(tmp, "rgb_in.dxt"), os.path.join(tmp, "rgb_out.rgb")
+    with open(src, "wb") as f:
+        f.write(four_colour_blocks(dxt))
+    run("decompress_rgb", src, w, h, out)
+    return np.frombuffer(read(out), np.uint8)
+
+
+def make_rgb():
+    """ref_rgb.npz: RGB8 known answers for the LoadRGB path, computed by the reference's
+    own CPU decompression (Decode565 + LerpChannels) through four_colour_blocks:
+      small_rgb          for ref_small.npz's file;
+      mixed_gtc / _dxt   a 512x256 file the reference encoder wrote from random starting
+                         blocks (seed 77 of numpy's default_rng) over the seed-22 texture,
+                         so that many blocks keep ep1 < ep2 or ep1 == ep2;
+      mixed_rgb_sha256   SHA-256 of its expected RGB8 (the pixels are noise and would not
+                         compress)."""
+    import hashlib
+    subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
+    with tempfile.TemporaryDirectory() as tmp:
+        small = np.load(os.path.join(HERE, "ref_small.npz"))
+        arrays = {"small_rgb": reference_rgb(small["dxt"].tobytes(), 512, 256, tmp)}
+        rgb = os.path.join(tmp, "mixed.rgb")
+        synthetic_rgb(22, 512, 256, 300, rgb)
+        blocks = os.path.join(tmp, "mixed.dxt_in")
+        with open(blocks, "wb") as f:
+            f.write(np.random.default_rng(77).integers(0, 256, 512 * 256 // 2, dtype=np.uint8).tobytes())
+        gtc, dxt = os.path.join(tmp, "mixed.gtc"), os.path.join(tmp, "mixed.dxt")
+        run("encode_rgb_dxt", rgb, 512, 256, blocks, gtc, dxt)
+        arrays["mixed_gtc"] = np.frombuffer(read(gtc), np.uint8)
+        arrays["mixed_dxt"] = np.frombuffer(read(dxt), np.uint8)
+        expect = reference_rgb(read(dxt), 512, 256, tmp)
+        arrays["mixed_rgb_sha256"] = np.frombuffer(hashlib.sha256(expect.tobytes()).digest(), np.uint8)
+    np.savez_compressed(os.path.join(HERE, "ref_rgb.npz"), **arrays)
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["hist"]:
         make_hist()
+    elif sys.argv[1:] == ["rgb"]:
+        make_rgb()
     else:
         main()
+        make_rgb()

@@ -111,17 +111,25 @@ def test_load_compressed_dxts_same_size_pages(device):
     assert device.download(d_out) == b"".join(dxts)
 
 
-@pytest.mark.parametrize("name", ["ref_c0", "ref_small"])
+@pytest.mark.parametrize("name", ["ref_c0", "ref_small", "mixed"])
 def test_load_rgb_matches_oracle(device, name):
-    g = load_golden(name)
-    gtc = g["gtc"].tobytes()
+    """LoadRGB against the oracle, and against the reference's own CPU decompression
+    (ref_rgb.npz) for ref_small and for a file with many blocks in three-colour order."""
+    import hashlib
+    rgbs = load_golden("ref_rgb")
+    gtc = (rgbs["mixed_gtc"] if name == "mixed" else load_golden(name)["gtc"]).tobytes()
     staging, (hdr,) = gst_amd.PackStaging([gtc])
     d_cmp = device.upload(staging)
     d_out = device.empty(hdr.width * hdr.height * 3)
     gst_amd.LoadRGB(hdr, device.stream, d_cmp.data_ptr(), d_out.data_ptr())
     rc, expected = go.decode_rgb(gtc)
     assert rc == go.OK
-    assert device.download(d_out) == expected
+    got = device.download(d_out)
+    assert got == expected
+    if name == "ref_small":
+        assert got == rgbs["small_rgb"].tobytes()
+    if name == "mixed":
+        assert hashlib.sha256(got).digest() == rgbs["mixed_rgb_sha256"].tobytes()
 
 
 @pytest.mark.parametrize("w,h,seed", [(512, 256, 21), (768, 512, 22), (1024, 1024, 23), (2048, 1024, 24),

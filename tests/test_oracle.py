@@ -255,3 +255,25 @@ def test_frequency_tables_must_sum_to_table_size():
     f = struct.unpack_from("<H", g, 28)[0]
     struct.pack_into("<H", g, 28, f + 1)
     assert go.decode_dxt(bytes(g))[0] == go.ERR_FREQ
+
+
+def test_rgb_output_matches_reference_decompression():
+    """gsto_decode_rgb (assemble_rgb, codec/assemble.cl:83-130) against the reference's
+    CPU decompression, DXTImage::DecompressedImage (codec/dxt_image.cpp:665-687), as
+    written to ref_rgb.npz by tests/golden/make_golden.py.  DecompressedImage follows the
+    DXT1 order rule and assemble_rgb always takes four colours, so the fixture fed the
+    reference blocks rewritten to keep every pixel's colour under the four-colour reading
+    (ep1 < ep2: endpoints swapped and indices' low bits flipped; ep1 == ep2: index 3 -> 0).
+    The mixed file has blocks of both kinds, decoded by the reference encoder's own
+    blocks."""
+    import hashlib
+    g = load_golden("ref_rgb")
+    small = load_golden("ref_small")
+    rc, rgb = go.decode_rgb(small["gtc"].tobytes(), nthreads=4)
+    assert rc == go.OK and rgb == g["small_rgb"].tobytes()
+    blk = np.frombuffer(g["mixed_dxt"].tobytes(), "<u2").reshape(-1, 4)
+    assert (blk[:, 0] < blk[:, 1]).sum() > 1000 and (blk[:, 0] == blk[:, 1]).any()
+    rc, dxt, _ = go.decode_dxt(g["mixed_gtc"].tobytes(), nthreads=4)
+    assert rc == go.OK and dxt == g["mixed_dxt"].tobytes()
+    rc, rgb = go.decode_rgb(g["mixed_gtc"].tobytes(), nthreads=4)
+    assert rc == go.OK and hashlib.sha256(rgb).digest() == g["mixed_rgb_sha256"].tobytes()
