@@ -68,7 +68,8 @@ def build_encoder():
 
 def build_cpp_tests():
     """tests/cpp/codec_test and tests/cpp/ans_ocl_test: the reference's end-to-end decoder
-    test and its ANS decoder tests against the C++ API."""
+    test and its ANS decoder tests against the C++ API; tests/cpp/encoder_test: the
+    encoder's C++ API."""
     outs = []
     for name in ("codec_test", "ans_ocl_test"):
         src = os.path.join(ROOT, "tests", "cpp", name + ".cpp")
@@ -76,6 +77,12 @@ def build_cpp_tests():
         outs.append(_atomic_build(out, lambda o, src=src: [
             HIPCC, "--offload-arch=gfx950", "-O2", "-std=c++17", "-Wall", "-Wno-unused-command-line-argument",
             "-I" + INCLUDE, src, "-L" + HERE, "-lgst_amd", "-Wl,-rpath," + HERE, "-o", o]))
+    # The encoder's C++ API needs no GPU: host compiler, libgst_encode.so only.
+    src = os.path.join(ROOT, "tests", "cpp", "encoder_test.cpp")
+    cxx = shutil.which("g++") or "g++"
+    outs.append(_atomic_build(os.path.join(ROOT, "tests", "cpp", "encoder_test"), lambda o: [
+        cxx, "-std=c++17", "-O2", "-Wall", "-Wextra", "-I" + INCLUDE, src, "-L" + HERE, "-lgst_encode",
+        "-Wl,-rpath," + HERE, "-o", o]))
     return outs
 
 

@@ -119,6 +119,26 @@ def test_argument_and_capacity_errors():
         encode.compress_dxt(rgb, dxt=b"\0" * 10)
 
 
+def test_cpp_encoder_api(tmp_path):
+    """tests/cpp/encoder_test.cpp: GenTC::CompressDXT through include/gst/encoder.hpp writes
+    the reference's file and blocks for the seed-0 texture; its second pass, from those
+    blocks, writes a file that decodes to the blocks it reports (and, where the reference
+    build is present, the file the reference writes from the same blocks)."""
+    exe = os.path.join(ROOT, "tests", "cpp", "encoder_test")
+    outs = [tmp_path / n for n in ("a.gtc", "a.dxt", "b.gtc", "b.dxt")]
+    proc = subprocess.run([exe, *map(str, outs)], capture_output=True, text=True, timeout=120)
+    assert proc.returncode == 0, proc.stdout + proc.stderr
+    g = load_golden("ref_c0")
+    assert outs[0].read_bytes() == g["gtc"].tobytes()
+    assert outs[1].read_bytes() == g["dxt"].tobytes()
+    rc, blocks, _ = go.decode_dxt(outs[2].read_bytes(), nthreads=4)
+    assert rc == go.OK and blocks == outs[3].read_bytes()
+    if os.path.exists(REF_DRIVER):
+        rgb = encode.generate_rgb(0, 512, 512, 300)
+        want = _reference(tmp_path, rgb, 512, 512, outs[1].read_bytes())
+        assert want == (outs[2].read_bytes(), outs[3].read_bytes())
+
+
 # ---- against the reference build, where it is present --------------------------
 
 def _reference(tmp_path, rgb, w, h, blocks=None):
