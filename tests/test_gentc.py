@@ -47,6 +47,21 @@ def test_reference_fixtures_byte_identical(name):
     assert gtc == g["gtc"].tobytes()
 
 
+def test_thread_pool_matches_single_calls_and_fixtures():
+    """gste_make_textures_gentenc (the benchmark's corpus path): largest first over a
+    thread pool, every file as gste_compress_dxt writes it, so the fixtures' textures come
+    out as the reference's files."""
+    specs = [(3, 512, 256), (0, 512, 512), (7, 1024, 512), (41, 256, 512)]
+    files, dxts = encode.make_textures(specs, nthreads=3, encoder="gentenc")
+    for name, i in (("ref_small", 0), ("ref_c0", 1), ("ref_wide", 2)):
+        g = load_golden(name)
+        assert files[i] == g["gtc"].tobytes() and dxts[i] == g["dxt"].tobytes(), name
+    gtc, dxt = encode.compress_dxt(encode.generate_rgb(41, 256, 512))
+    assert files[3] == gtc and dxts[3] == dxt
+    with pytest.raises(ValueError):
+        encode.make_textures(specs[:1], encoder="other")
+
+
 def test_oracle_decodes_to_the_encoder_blocks():
     for seed, w, h in ((41, 512, 256), (42, 256, 1024)):
         gtc, dxt = encode.compress_dxt(encode.generate_rgb(seed, w, h, 500))
