@@ -209,7 +209,7 @@ def test_reference_build_also_refuses_a_zero_stream(tmp_path):
 # ---- GPU ------------------------------------------------------------------------
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("seed,w,h", [(0, 512, 512), (31, 512, 256), (32, 1024, 1024)])
+@pytest.mark.parametrize("seed,w,h", [(0, 512, 512), (31, 512, 256), (32, 1024, 1024), (33, 2048, 2048)])
 def test_encoder_files_decode_on_gpu(device, seed, w, h):
     """Files the restated encoder writes on this host decode on the device to its
     blocks; the 512x512 seed-0 file is the committed fixture written by the reference."""
