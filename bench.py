@@ -429,8 +429,10 @@ def main():
 
     t0 = time.perf_counter()
     specs = corpus_c3(args.images)
-    files, dxts = encode.make_textures(specs, nthreads=min(16, os.cpu_count() or 1), encoder="gentenc")
-    log("corpus: %d textures encoded in %.1fs" % (len(files), time.perf_counter() - t0))
+    enc_threads = min(16, os.cpu_count() or 1)
+    files, dxts = encode.make_textures(specs, nthreads=enc_threads, encoder="gentenc")
+    enc_seconds = time.perf_counter() - t0
+    log("corpus: %d textures encoded in %.1fs" % (len(files), enc_seconds))
 
     hdrs, ms, stage, dev_bytes, tuned, launch = run_batched(gpu, files, dxts, args.steps, args.warmup,
                                                             tune=not args.no_tune)
@@ -471,6 +473,8 @@ def main():
             "compressed_bytes": sum(len(f) for f in files),
             "plan_device_bytes": dev_bytes,
             "parallelism": "single GPU",
+            "corpus_encode": {"encoder": "gste_compress_dxt (byte-identical to gentenc), CPU",
+                              "threads": enc_threads, "seconds": round(enc_seconds, 2)},
         },
         "roofline": {
             "bound": "hbm",
