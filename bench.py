@@ -47,22 +47,26 @@ def corpus_c3(n=128, size_seed=42):
     return [(1000 + i, SIZES_C3[p][0], SIZES_C3[p][1]) for i, p in enumerate(picks)]
 
 
-def stage_bytes(hdrs, mode="tile_fused"):
-    """Algorithmic read+write bytes per stage for one decode of `hdrs`.  `mode` is how
-    the last stages run: "tile_fused" (wavelet, index scan and DXT1 assembly in one
-    kernel: the int8 planes stay in LDS), "fused" (index scan inside the assembly:
-    it reads the nv diff bytes instead of nv int32 indices) or "separate".  The rANS
-    stage also writes the index stream's partial sums (nv/64 + nv/8 bytes), which the
-    fused assemblies read."""
+def stage_bytes(hdrs, mode="tile_fused", tables_in_lds=True):
+    """Algorithmic read+write bytes per stage for one decode of `hdrs`; 0 for a stage
+    with no launch of its own.  `mode` is how the last stages run: "tile_fused"
+    (wavelet, index scan and DXT1 assembly in one kernel: the int8 planes stay in LDS),
+    "fused" (index scan inside the assembly: it reads the nv diff bytes instead of nv
+    int32 indices) or "separate".  With `tables_in_lds` the rANS kernel builds its
+    tables from the four 512-byte frequency tables of each image (2048 bytes read, no
+    table traffic); otherwise build_table writes 4 x 8 KiB tables that the rANS stage
+    reads.  The rANS stage also writes the index stream's partial sums (nv/64 + nv/8
+    bytes), which the fused assemblies read."""
     nv = sum(h.num_blocks for h in hdrs)
     P = sum(h.palette_bytes for h in hdrs)
     S = sum(h.stream_bytes for h in hdrs)
     n = len(hdrs)
     partial = nv // 64 + nv // 8
     scan_folded = mode in ("tile_fused", "fused")
+    table_read = 2048 * n if tables_in_lds else 32768 * n
     return {
-        "build_table": 2048 * n + 32768 * n,
-        "ans_decode": (S + 32768 * n) + (7 * nv + P) + partial,
+        "build_table": 0 if tables_in_lds else 2048 * n + 32768 * n,
+        "ans_decode": (S + table_read) + (7 * nv + P) + partial,
         "decode_indices": 0 if scan_folded else nv + 4 * nv,
         "inv_wavelet": 0 if mode == "tile_fused" else 6 * nv + 6 * nv,
         "assemble": ((6 * nv + nv + partial + P) if scan_folded else (6 * nv + 4 * nv + P)) + 8 * nv,
@@ -82,10 +86,9 @@ def measured_traffic(stage, n_images, requested):
 
 
 def pipeline_bytes(hdrs, mode="tile_fused", tables_in_lds=True):
-    """B_pipe: the stages' algorithmic bytes in `mode` (the table stage's only when it
-    has a launch of its own); B_min: inputs read once (streams, frequencies) and the
-    DXT1 output written once."""
-    b_pipe = sum(v for k, v in stage_bytes(hdrs, mode).items() if k != "build_table" or not tables_in_lds)
+    """B_pipe: the algorithmic bytes of the kernels the decode launches in `mode`;
+    B_min: inputs read once (streams, frequencies) and the DXT1 output written once."""
+    b_pipe = sum(stage_bytes(hdrs, mode, tables_in_lds).values())
     b_min = sum((h.stream_bytes + 2048) + 8 * h.num_blocks for h in hdrs)
     return b_pipe, b_min
 
@@ -218,10 +221,11 @@ def run_batched(gpu, files, dxts, steps, warmup, tune=True):
     # Per-stage kernel time in the pipeline's order and cache state: the five launches
     # with a hipEvent recorded on the decode stream before and after each, mean of 20.
     reps = 20
-    stage = {k: 0.0 for k in gst_amd.STAGES}
+    stage = {k: None for k in gst_amd.STAGES}
     for _ in range(reps):
         for k, v in b.decode_timed(gpu.stream).items():
-            stage[k] += v / reps
+            if v is not None:
+                stage[k] = (stage[k] or 0.0) + v / reps
     check_outputs(gpu, d_out, out_offs, dxts, "batched timed")
     dev_bytes = b.device_bytes
     b.close()
@@ -259,9 +263,29 @@ def run_plain(gpu, files, dxts, passes, warmup):
     return hdrs, float(np.median(times))
 
 
+def cgroup_cpu_quota():
+    """CPUs granted by the cgroup's CPU quota (cgroup v2 cpu.max), or None."""
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            quota, period = f.read().split()[:2]
+        return None if quota == "max" else max(1, int(int(quota) / int(period)))
+    except (OSError, ValueError):
+        return None
+
+
+def host_cores():
+    """(os.cpu_count(), the cores this process may use: its CPU affinity, capped by the
+    cgroup CPU quota when there is one)."""
+    usable = len(os.sched_getaffinity(0))
+    quota = cgroup_cpu_quota()
+    return os.cpu_count() or 1, min(usable, quota) if quota else usable
+
+
 def run_cpu_baseline(files, min_seconds=10.0, max_passes=50):
+    """The oracle (the C restatement of the five kernels) decoding the whole corpus on
+    every core this process may use, until at least `min_seconds` have passed."""
     from oracle import gst_oracle as go
-    threads = min(16, os.cpu_count() or 1)
+    threads = host_cores()[1]
     texels = 0
     for f in files:
         rc, h = go.parse_header(f)
@@ -279,14 +303,33 @@ def run_cpu_baseline(files, min_seconds=10.0, max_passes=50):
     return float(np.median(rates)), threads, len(rates), t_total
 
 
+def run_cpu_single(f, expected, runs=7):
+    """One texture decoded by the oracle on every core: a warm-up, then the median of
+    `runs` timed decodes, the output checked against the expected blocks."""
+    from oracle import gst_oracle as go
+    threads = host_cores()[1]
+    rc, out, _ = go.decode_dxt(f, nthreads=threads)
+    if rc != go.OK or out != expected:
+        raise SystemExit("CPU baseline: the oracle's decode differs from the expected blocks")
+    times = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        go.decode_dxt(f, nthreads=threads)
+        times.append(time.perf_counter() - t0)
+    return float(np.median(times)) * 1e3, threads
+
+
 SERVED_MIX = [((512, 512), 0.50, 16), ((1024, 1024), 0.30, 12), ((2048, 2048), 0.15, 6),
               ((4096, 4096), 0.05, 2)]
 
 
-def run_served(qps_list, duration_s, budget_gib=32, closed_loop=64, tuned=None):
+def run_served(gpu, qps_list, duration_s, budget_gib=32, closed_loop=512, tuned=None):
     """Poisson arrivals of the size mix through the continuous-batching server and
-    through the plain per-request server; latency is arrival -> output in HBM.  The
-    batched server takes the rANS launch shape the batched leg's autotune chose."""
+    through the plain per-request server, then saturating closed loops.  Latency is
+    arrival -> output in HBM (and -> copied back to pinned host memory).  Every
+    completion is compared with its expected blocks on the device (the pool's expected
+    outputs stay in HBM); a failure or a difference ends the benchmark.  The batched
+    server takes the rANS launch shape the batched leg's autotune chose."""
     shape = {}
     if tuned:
         shape = {"groups_per_wave": tuned["ans_groups_per_wave"], "waves_per_group": tuned["ans_waves_per_group"]}
@@ -299,15 +342,23 @@ def run_served(qps_list, duration_s, budget_gib=32, closed_loop=64, tuned=None):
             specs.append((seed, w, h))
             weights.append(p / count)
             seed += 1
-    pool, _ = encode.make_textures(specs, want_dxt=False, nthreads=min(16, os.cpu_count() or 1),
-                                   encoder="gentenc")
-    out = {"pool": "%d pre-encoded textures, size mix {512^2: 50%%, 1024^2: 30%%, 2048^2: 15%%, 4096^2: 5%%}"
-                   % len(pool), "hbm_budget_GiB": budget_gib, "duration_s_per_point": duration_s}
+    pool, dxts = encode.make_textures(specs, nthreads=host_cores()[1], encoder="gentenc")
+    d_pool, pool_dev = gpu.upload_many(pool)
+    d_exp, pool_exp = gpu.upload_many(dxts)
+    gpu.torch.cuda.synchronize()
+    out = {"pool": "%d pre-encoded textures (gentenc bytes), size mix {512^2: 50%%, 1024^2: 30%%, 2048^2: 15%%, "
+                   "4096^2: 5%%}" % len(pool),
+           "hbm_budget_GiB": budget_gib, "duration_s_per_point": duration_s,
+           "checked": "every completion compared with its expected DXT1 blocks on the device"}
 
-    def summarize(lat_dev, lat_host, n, wall, st):
+    def summarize(lat_dev, lat_host, res, st):
+        if res["failed"] or res["mismatches"] or res["verified"] != res["requests"]:
+            raise SystemExit("served: %s" % res)
         busy = st["device_busy_ms"] * 1e-3
+        wall = res["wall_s"]
         return {
-            "requests": int(n),
+            "requests": int(res["requests"]),
+            "outputs_checked": int(res["verified"]),
             "p50_us": round(float(np.percentile(lat_dev, 50)), 1),
             "p99_us": round(float(np.percentile(lat_dev, 99)), 1),
             "p50_with_d2h_us": round(float(np.percentile(lat_host, 50)), 1),
@@ -317,38 +368,49 @@ def run_served(qps_list, duration_s, budget_gib=32, closed_loop=64, tuned=None):
             "busy_frac": round(busy / wall, 4),
             "B_pipe_roofline_busy": round(st["algorithmic_bytes"] / busy / (HBM_PEAK_GBS * 1e9), 4)
             if busy > 0 else None,
+            "B_pipe_roofline_wall": round(st["algorithmic_bytes"] / wall / (HBM_PEAK_GBS * 1e9), 4),
             "mean_batch_images": round(st["mean_batch_images"], 2),
             "max_batch_images": int(st["max_batch_images"]),
+            "h2d_GB": round(st["h2d_bytes"] / 1e9, 3),
+            "h2d_copies": int(st["h2d_copies"]),
+            "scheduler_launch_ms": round(st["launch_host_ms"], 1),
         }
 
-    budget = budget_gib << 30
+    def one(label, server_kw, load_kw):
+        with Server(budget_gib << 30, **server_kw) as s:
+            s.run_load(pool[:4], qps=200.0, duration_s=0.2, seed=1, pool_expected=pool_exp[:4])  # warm-up
+            base = s.stats()
+            lat_dev, lat_host, res = s.run_load(pool, weights, pool_expected=pool_exp, duration_s=duration_s,
+                                                **load_kw)
+            st = s.stats()
+        for k in ("texels", "algorithmic_bytes", "h2d_bytes", "h2d_copies"):
+            st[k] -= base[k]
+        st["device_busy_ms"] -= base["device_busy_ms"]
+        st["launch_host_ms"] -= base["launch_host_ms"]
+        r = summarize(lat_dev, lat_host, res, st)
+        log("served %s: %s" % (label, r))
+        return r
+
     for mode in ("batched", "plain"):
         res = {}
         for qps in qps_list:
-            with Server(budget, plain=(mode == "plain"), copy_to_host=True, auto_release=True,
-                        **(shape if mode == "batched" else {})) as s:
-                s.run_load(pool[:4], qps=200.0, duration_s=0.2, seed=1)  # warm-up (graphs, pools)
-                base = s.stats()
-                lat_dev, lat_host, n, wall = s.run_load(pool, weights, qps=qps, duration_s=duration_s,
-                                                        seed=int(qps))
-                st = s.stats()
-                for k in ("texels", "algorithmic_bytes"):
-                    st[k] -= base[k]
-                st["device_busy_ms"] -= base["device_busy_ms"]
-                res[str(int(qps))] = summarize(lat_dev, lat_host, n, wall, st)
-            log("served %s @%d req/s: %s" % (mode, qps, res[str(int(qps))]))
+            kw = dict(plain=(mode == "plain"), copy_to_host=True, **(shape if mode == "batched" else {}))
+            res[str(int(qps))] = one("%s @%d req/s" % (mode, qps), kw, dict(qps=qps, seed=int(qps)))
         out[mode] = res
-    with Server(budget, copy_to_host=False, auto_release=True, **shape) as s:
-        s.run_load(pool[:4], qps=200.0, duration_s=0.2, seed=1)
-        base = s.stats()
-        lat_dev, lat_host, n, wall = s.run_load(pool, weights, closed_loop=closed_loop, duration_s=duration_s,
-                                                seed=7)
-        st = s.stats()
-        for k in ("texels", "algorithmic_bytes"):
-            st[k] -= base[k]
-        st["device_busy_ms"] -= base["device_busy_ms"]
-        out["saturated_closed_loop_%d" % closed_loop] = summarize(lat_dev, lat_host, n, wall, st)
-        log("served saturated: %s" % out["saturated_closed_loop_%d" % closed_loop])
+    # Saturating closed loops: host-submitted inputs (pinned staging -> copy stream ->
+    # HBM, the ingest path), the same with the pool pre-staged in HBM (decode and
+    # scheduling only), and the plain per-request server.
+    out["closed_loop"] = {
+        "depth": closed_loop,
+        "batched_ingest": one("closed loop, ingest", dict(shape), dict(closed_loop=closed_loop, seed=7)),
+        "batched_prestaged": one("closed loop, pre-staged", dict(shape),
+                                 dict(closed_loop=closed_loop, seed=7, pool_device=pool_dev)),
+        "plain_ingest": one("closed loop, plain", dict(plain=True), dict(closed_loop=64, seed=7)),
+    }
+    cl = out["closed_loop"]
+    cl["speedup_prestaged_vs_plain"] = round(cl["batched_prestaged"]["Mtexel_s"] / cl["plain_ingest"]["Mtexel_s"], 2)
+    cl["speedup_ingest_vs_plain"] = round(cl["batched_ingest"]["Mtexel_s"] / cl["plain_ingest"]["Mtexel_s"], 2)
+    del d_pool, d_exp
     return out
 
 
@@ -429,7 +491,7 @@ def main():
 
     t0 = time.perf_counter()
     specs = corpus_c3(args.images)
-    enc_threads = min(16, os.cpu_count() or 1)
+    enc_threads = host_cores()[1]
     files, dxts = encode.make_textures(specs, nthreads=enc_threads, encoder="gentenc")
     enc_seconds = time.perf_counter() - t0
     log("corpus: %d textures encoded in %.1fs" % (len(files), enc_seconds))
@@ -444,10 +506,12 @@ def main():
         mode = "tile_fused"
     else:
         mode = "fused" if tuned["assembly_waves_per_group"] > 0 else "separate"
-    sb = stage_bytes(hdrs, mode)
-    dom = max(stage, key=stage.get)
+    lds_tables = tuned is None or tuned["ans_builds_tables"] == 1
+    sb = stage_bytes(hdrs, mode, lds_tables)
+    launched = {k: v for k, v in stage.items() if v is not None}
+    dom = max(launched, key=launched.get)
     achieved = sb[dom] / (stage[dom] * 1e-3) / 1e9
-    b_pipe, b_min = pipeline_bytes(hdrs, mode, tables_in_lds=tuned is None or tuned["ans_builds_tables"] == 1)
+    b_pipe, b_min = pipeline_bytes(hdrs, mode, tables_in_lds=lds_tables)
     result = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -493,13 +557,16 @@ def main():
         },
         "autotune": tuned,
         "launch_ms_per_step": {k: round(v, 4) for k, v in launch.items()},
-        "stage_ms": {k: round(v, 4) for k, v in stage.items()},
-        "stage_gbs": {k: round(sb[k] / (stage[k] * 1e-3) / 1e9, 1) if sb[k] else None for k in stage},
+        "stage_ms": {k: (round(v, 4) if v is not None else None) for k, v in stage.items()},
+        "stage_gbs": {k: (round(sb[k] / (v * 1e-3) / 1e9, 1) if v is not None and sb[k] else None)
+                      for k, v in stage.items()},
+        "stage_note": "null: the stage has no launch of its own in the tuned shapes",
         "last_stages": {"tile_fused": "wavelet, index scan and DXT1 assembly in one kernel",
                         "fused": "wavelet launch, then index scan inside the assembly",
                         "separate": "wavelet, index scan and assembly launches"}[mode],
     }
 
+    c2 = None
     if not args.no_plain:
         _, ms_plain = run_plain(gpu, files, dxts, passes=3, warmup=1)
         plain = texels / (ms_plain * 1e-3) / 1e6
@@ -513,6 +580,8 @@ def main():
             (f,), (d,) = encode.make_textures([(seed, w, h)], encoder="gentenc")
             _, ms_s = run_plain(gpu, [f], [d], passes=5, warmup=2)
             singles[name] = {"ms": round(ms_s, 4), "Mtexel_s": round(w * h / (ms_s * 1e-3) / 1e6, 1)}
+            if name.startswith("C2"):
+                c2 = (f, d)
         result["single_image"] = singles
         log("single images: %s" % singles)
 
@@ -520,17 +589,26 @@ def main():
         result["ans_api"] = run_ans_api(files)
 
     if not args.no_served:
-        result["served"] = run_served([200.0, 1000.0, 5000.0], args.served_seconds, tuned=tuned)
+        result["served"] = run_served(gpu, [200.0, 1000.0, 5000.0], args.served_seconds, tuned=tuned)
 
     if not args.no_cpu:
         rate, threads, passes, secs = run_cpu_baseline(files)
+        cpu_count, usable = host_cores()
         result["cpu_baseline"] = {
             "value": round(rate, 1), "unit": "Mtexel/s", "cores": threads, "kind": "port",
+            "host_cpu_count": cpu_count, "host_cores_usable": usable,
+            "host_cpu_affinity": len(os.sched_getaffinity(0)), "host_cgroup_cpu_quota": cgroup_cpu_quota(),
             "sample": "the full C3 corpus (%d textures, %.0f Mtexel) decoded %d times in %.1f s by the C "
-                      "restatement, one task per rANS group / wavelet tile / block chunk" %
-                      (len(files), texels / 1e6, passes, secs),
+                      "restatement on %d threads (every core this process may use), one task per rANS group / "
+                      "wavelet tile / block chunk" % (len(files), texels / 1e6, passes, secs, threads),
         }
-        log("cpu baseline: %.0f Mtexel/s on %d threads" % (rate, threads))
+        log("cpu baseline: %.0f Mtexel/s on %d threads (cpu_count %d)" % (rate, threads, cpu_count))
+        if c2 is not None:
+            ms_cpu, threads = run_cpu_single(*c2)
+            result["cpu_baseline"]["C2_4096x4096"] = {
+                "ms": round(ms_cpu, 2), "Mtexel_s": round(4096 * 4096 / (ms_cpu * 1e-3) / 1e6, 1),
+                "threads": threads, "gpu_ms": result["single_image"]["C2_4096x4096"]["ms"]}
+            log("cpu C2: %.2f ms on %d threads" % (ms_cpu, threads))
     print(json.dumps(result), flush=True)
 
 

@@ -16,6 +16,7 @@ raises GstError on a non-zero status.  The library must be built
 (gst_amd/build.py); importing without it raises, there is no fallback path.
 """
 import ctypes
+import math
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -85,6 +86,11 @@ class BatchConfig(ctypes.Structure):
     ]
 
 
+def _stage_times(ms):
+    """Stage name -> ms, None for a stage with no launch of its own (NaN from the ABI)."""
+    return {k: (None if math.isnan(v) else v) for k, v in zip(STAGES, list(ms))}
+
+
 class TuneResult(ctypes.Structure):
     _fields_ = [(name, ctypes.c_uint32) for name in (
         "ans_groups_per_wave", "ans_waves_per_group", "wavelet_waves_per_group", "assembly_waves_per_group",
@@ -94,7 +100,7 @@ class TuneResult(ctypes.Structure):
 
     def as_dict(self):
         d = {name: getattr(self, name) for name, _ in self._fields_[:11]}
-        d["stage_ms"] = dict(zip(STAGES, list(self.stage_ms)))
+        d["stage_ms"] = _stage_times(self.stage_ms)
         d["total_ms"] = self.total_ms
         d["wavelet_fused"] = self.wavelet_fused
         d["streams_per_wavefront"] = 32 * self.ans_groups_per_wave
@@ -352,7 +358,7 @@ class Batch:
     def decode_timed(self, stream):
         ms = (ctypes.c_float * 5)()
         _check(lib().gst_batch_decode_timed(self._h, stream, ms), "gst_batch_decode_timed")
-        return dict(zip(STAGES, list(ms)))
+        return _stage_times(ms)
 
     def error_bits(self, stream):
         bits = _U32()

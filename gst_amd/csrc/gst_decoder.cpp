@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <limits>
 #include <atomic>
 #include <cstring>
 #include <map>
@@ -470,6 +471,16 @@ int gst_read_error_bits(gst_stream stream, uint32_t *bits) {
   GST_HIP(hipStreamSynchronize(s));
   GST_HIP(hipMemcpy(bits, w, sizeof(uint32_t), hipMemcpyDeviceToHost));
   GST_HIP(hipMemset(w, 0, sizeof(uint32_t)));
+  return GST_OK;
+}
+
+int gst_read_error_bits_async(gst_stream stream, uint32_t *host_bits) {
+  if (!host_bits) return GST_ERR_INVALID_ARGUMENT;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  uint32_t *w = ErrWord(s);
+  if (!w) return GST_ERR_HIP;
+  GST_HIP(hipMemcpyAsync(host_bits, w, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  GST_HIP(hipMemsetAsync(w, 0, sizeof(uint32_t), s));
   return GST_OK;
 }
 
@@ -959,7 +970,12 @@ int gst_batch_autotune(gst_batch *b, gst_stream stream, uint32_t reps, gst_tune_
   res.grid_indices = grids[gst::kStageIndices];
   res.grid_wavelet = grids[gst::kStageWavelet];
   res.grid_assemble = grids[gst::kStageAssemble];
-  for (float t : res.stage_ms) res.total_ms += t;
+  for (uint32_t st = 0; st < gst::kNumStages; ++st) {
+    if (gst::StageLaunches(st, grids[st], best))
+      res.total_ms += res.stage_ms[st];
+    else
+      res.stage_ms[st] = std::numeric_limits<float>::quiet_NaN();
+  }
   if (out) *out = res;
   return GST_OK;
 }
@@ -994,8 +1010,12 @@ int gst_batch_decode_timed(gst_batch *b, gst_stream stream, float ms[5]) {
     if (e == hipSuccess) e = hipEventRecord(ev[st + 1], s);
   }
   if (e == hipSuccess) e = hipEventSynchronize(ev[gst::kNumStages]);
-  for (uint32_t st = 0; st < gst::kNumStages && e == hipSuccess; ++st)
-    e = hipEventElapsedTime(&ms[st], ev[st], ev[st + 1]);
+  for (uint32_t st = 0; st < gst::kNumStages && e == hipSuccess; ++st) {
+    if (gst::StageLaunches(st, b->grids[st], b->stage))
+      e = hipEventElapsedTime(&ms[st], ev[st], ev[st + 1]);
+    else
+      ms[st] = std::numeric_limits<float>::quiet_NaN();  // nothing launched: no time to report
+  }
   if (e != hipSuccess) rc = HipFail(e);
   for (auto &x : ev) (void)hipEventDestroy(x);
   return rc;

@@ -13,7 +13,7 @@
 //   palette = u32 DXT1 index words, zero-padded to 8192 B; idx = (i_k - i_{k-1}) + 128
 //             (codec/dxt_image.cpp:597-619)
 // The rANS parameters are the reference decoder's: M = 2^11, L = 2^15, 16-bit
-// renormalisation (ans/ans_ocl_encode.cpp:10-18, ans/ans_decode.cl:3-8).
+// renormalisation (ans/ans_ocl_encode.cpp:10-18, ans/ans_decode.cl:4-8).
 #include "gst_encode.h"
 
 #include <algorithm>
@@ -232,7 +232,7 @@ void LoadBlockPixels(const uint8_t *rgb, uint32_t width, uint32_t bx, uint32_t b
 
 // ---------------------------------------------------------------------------
 // Forward 5/3 lifting, the exact integer inverse of the decoder's inverse step
-// (codec/inverse_wavelet.cl:112-148): high-pass first, then low-pass update.
+// (codec/inverse_wavelet.cl:28-64): high-pass first, then low-pass update.
 
 int Mirror(int idx, int range) {
   const int v = idx - static_cast<int>(idx >= range) * (idx - range + 2);

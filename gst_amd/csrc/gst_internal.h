@@ -37,7 +37,7 @@ enum DevErr : uint32_t {
 // Per-image descriptor.  Stream k: 0 = Y, 1 = chroma, 2 = palette, 3 = indices
 // (codec/encoder.cpp:122-144).  Symbol buffer layout (codec/decoder.cpp:441-461):
 // [Y 2nv][chroma 4nv][palette P][indices nv]; planes p = Y1,Y2,Co1,Cg1,Co2,Cg2 are the
-// first 6nv symbols (codec/assemble.cl:303-313).
+// first 6nv symbols (codec/assemble.cl:27-37).
 struct ImageDesc {
   uint32_t width, height, nv, bx;
   uint32_t palette_bytes;
@@ -153,6 +153,17 @@ inline uint32_t SinglePassGrid(uint32_t stage, uint32_t units, const StageConfig
   return units;
 }
 
+// Whether LaunchStage puts a kernel on the stream for this stage: a grid of 0, tables
+// built inside the rANS kernel, and an index scan or wavelet folded into the assembly
+// launch nothing.
+inline bool StageLaunches(uint32_t stage, uint32_t grid, const StageConfig &c) {
+  if (grid == 0) return false;
+  if (stage == kStageTables) return !c.ans_build_tables;
+  if (stage == kStageIndices) return !c.fused_waves && !c.tile_fused;
+  if (stage == kStageWavelet) return !c.tile_fused;
+  return true;
+}
+
 // Launches one stage with `grid` work-groups (each loops over units grid-stride).
 hipError_t LaunchStage(uint32_t stage, const KernelArgs &args, uint32_t grid,
                        const StageConfig &cfg, hipStream_t stream);
@@ -161,6 +172,23 @@ hipError_t LaunchStage(uint32_t stage, const KernelArgs &args, uint32_t grid,
 // interleaved streams each) against one packed table; out gets groups * n * 256 bytes.
 hipError_t LaunchAnsGroups(const uint32_t *table, const uint8_t *data, uint32_t data_len, uint32_t groups,
                            uint32_t n, uint8_t *out, uint32_t *err, hipStream_t stream);
+
+// Device-side comparison of decoded outputs with expected bytes: the server checks
+// every completion this way under load.  Job i sets flags[job.flag] to 1 when its
+// `bytes` (a multiple of 16, both pointers 16-byte aligned) differ anywhere.
+struct CompareJob {
+  const uint8_t *out;
+  const uint8_t *expected;
+  uint64_t bytes;
+  uint64_t flag;
+};
+constexpr uint32_t kCompareJobs = 64;
+struct CompareArgs {
+  CompareJob job[kCompareJobs];
+  uint32_t n;
+  uint32_t *flags;
+};
+hipError_t LaunchCompare(const CompareArgs &a, hipStream_t stream);
 
 // The decoder library's one-time device set-up (gst_initialize_decoder(0) on first use).
 int EnsureDecoderInit();

@@ -220,7 +220,7 @@ __global__ __launch_bounds__(256) void BuildTableKernel(KernelArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Stage 2: interleaved rANS decode (ans/ans_decode.cl:97-157).
+// Stage 2: interleaved rANS decode (ans/ans_decode.cl:25-74,97-143).
 //
 // A work-group of WAVES wave64s decodes GPW*WAVES consecutive groups of one stream
 // with that stream's table resident in LDS in the 4-byte packed form BuildTableKernel
@@ -800,7 +800,7 @@ __global__ __launch_bounds__(256) void DecodeIndicesKernel(KernelArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Stage 4: inverse 5/3 wavelet (codec/inverse_wavelet.cl:98-276).
+// Stage 4: inverse 5/3 wavelet (codec/inverse_wavelet.cl:14-192).
 //
 // A wave64 transforms four 32x32 tiles (two work units) at once: lane r of half-wave
 // h holds row r of tile h of the first unit in the low 16 bits and row r of tile h
@@ -1011,7 +1011,7 @@ __global__ __launch_bounds__(WAVES * 64) void InvWaveletKernel(KernelArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Stage 5: DXT1 assembly (codec/assemble.cl:290-357) and RGB8 assembly (:359-406).
+// Stage 5: DXT1 assembly (codec/assemble.cl:14-81) and RGB8 assembly (:83-130).
 //
 // A thread assembles 4 consecutive blocks: one 4-byte load per plane, one 16-byte
 // index load, 4 palette gathers and two 16-byte stores.  YCoCg -> RGB565 uses C
@@ -1536,6 +1536,29 @@ hipError_t LaunchStage(uint32_t stage, const KernelArgs &args, uint32_t grid,
     default:
       return hipErrorInvalidValue;
   }
+}
+
+// One work-group row per job: 16-byte loads grid-stride over the job's bytes, one
+// ballot per wave, and a flag store only where a difference was seen.
+__global__ __launch_bounds__(256) void CompareKernel(CompareArgs a) {
+  const CompareJob j = a.job[blockIdx.y];
+  const GPtr<const uint4> x = Gbl<const uint4>(j.out);
+  const GPtr<const uint4> y = Gbl<const uint4>(j.expected);
+  const uint64_t words = j.bytes / 16;
+  bool diff = false;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < words;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const uint4 p = x[i], q = y[i];
+    diff |= ((p.x ^ q.x) | (p.y ^ q.y) | (p.z ^ q.z) | (p.w ^ q.w)) != 0;
+  }
+  if (__builtin_amdgcn_ballot_w64(diff) != 0 && __lane_id() == 0) Gbl<uint32_t>(a.flags)[j.flag] = 1u;
+}
+
+hipError_t LaunchCompare(const CompareArgs &a, hipStream_t stream) {
+  if (a.n == 0) return hipSuccess;
+  if (a.n > kCompareJobs) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(CompareKernel, dim3(64, a.n), dim3(256), 0, stream, a);
+  return hipGetLastError();
 }
 
 hipError_t LaunchAnsGroups(const uint32_t *table, const uint8_t *data, uint32_t data_len, uint32_t groups,

@@ -15,8 +15,8 @@
  *                 (codec/decoder.cpp:109-209, demo/photos_sf.cpp:749-795); for n = 1
  *                 this is the file minus its header at offset 512
  *   output      = n x (w*h/2) bytes of DXT1 blocks in raster block order
- *                 (codec/assemble.cl:340-357), or n x (w*h*3) RGB8 bytes
- *                 (codec/assemble.cl:359-406)
+ *                 (codec/assemble.cl:64-81), or n x (w*h*3) RGB8 bytes
+ *                 (codec/assemble.cl:83-130)
  */
 #ifndef GST_DECODER_H_
 #define GST_DECODER_H_
@@ -113,6 +113,11 @@ int gst_free_decompressor(void);
  * stream's queued work completes (blocking).  Async entry points report corrupt
  * input only through this call. */
 int gst_read_error_bits(gst_stream stream, uint32_t *bits);
+/* Non-blocking form (no reference counterpart): enqueues on `stream` a copy of the
+ * stream's error bits into `host_bits` (pinned host memory) and then their reset, so
+ * each decode on the stream can be given its own bits; read `host_bits` once the
+ * stream has passed that point. */
+int gst_read_error_bits_async(gst_stream stream, uint32_t *host_bits);
 
 /* ---- staging layout helpers (host) ------------------------------------------- */
 
@@ -129,7 +134,7 @@ int gst_pack_staging(const uint8_t *const *files, const size_t *lens, uint32_t n
  * d_streams points at the image's stream section (Y stream first).  Scratch layout:
  * d_tables 4 x 2048 u32 plus 4 u32 status words; d_symbols 7*nv+P bytes in the
  * reference's ANS output layout [Y 2nv][chroma 4nv][palette P][indices nv]
- * (codec/decoder.cpp:441-461, ans/ans_decode.cl:154-155); d_group_sums nv/8192
+ * (codec/decoder.cpp:441-461, ans/ans_decode.cl:71-72); d_group_sums nv/8192
  * int32; d_planes 6*nv int8 raster planes Y1,Y2,Co1,Cg1,Co2,Cg2; d_indices nv int32;
  * d_err one u32 of GST_DEVERR_* bits, which the stages only OR into: the caller
  * zeroes it before the first stage and reads it after the last. */
@@ -144,14 +149,14 @@ int gst_stage_ans_decode(const gst_header *hdr, const void *d_streams, const uin
 /* codec/decode_indices.cl:6-84 (codec/decoder.cpp:302-393). */
 int gst_stage_decode_indices(const gst_header *hdr, const uint8_t *d_symbols,
                              const int32_t *d_group_sums, int32_t *d_indices, gst_stream stream);
-/* codec/inverse_wavelet.cl:153-276 (codec/decoder.cpp:242-300). */
+/* codec/inverse_wavelet.cl:69-192 (codec/decoder.cpp:242-300). */
 int gst_stage_inv_wavelet(const gst_header *hdr, const uint8_t *d_symbols, int8_t *d_planes,
                           gst_stream stream);
-/* codec/assemble.cl:340-357 (codec/decoder.cpp:395-417). */
+/* codec/assemble.cl:64-81 (codec/decoder.cpp:395-417). */
 int gst_stage_assemble_dxt(const gst_header *hdr, const uint8_t *d_symbols, const int8_t *d_planes,
                            const int32_t *d_indices, void *d_out, uint32_t *d_err,
                            gst_stream stream);
-/* codec/assemble.cl:359-406. */
+/* codec/assemble.cl:83-130. */
 int gst_stage_assemble_rgb(const gst_header *hdr, const uint8_t *d_symbols, const int8_t *d_planes,
                            const int32_t *d_indices, void *d_out, uint32_t *d_err,
                            gst_stream stream);
@@ -201,7 +206,9 @@ int gst_batch_decode(gst_batch *b, gst_stream stream);
 int gst_batch_capture_graph(gst_batch *b, gst_stream stream);
 int gst_batch_launch_graph(gst_batch *b, gst_stream stream);
 /* Per-stage kernel time (ms) of one decode, measured with hipEvents on `stream`:
- * build_table, ans_decode, decode_indices, inv_wavelet, assemble. Blocking. */
+ * build_table, ans_decode, decode_indices, inv_wavelet, assemble.  A stage that
+ * launches nothing in the plan's shape (tables built inside the rANS kernel, an index
+ * scan or wavelet folded into the assembly) reports NaN.  Blocking. */
 int gst_batch_decode_timed(gst_batch *b, gst_stream stream, float ms[5]);
 int gst_batch_error_bits(gst_batch *b, gst_stream stream, uint32_t *bits);
 /* Non-blocking form: enqueues on `stream` a copy of the error bits into `host_bits`
@@ -217,7 +224,8 @@ typedef struct gst_tune_result {
   uint32_t assembly_wide;            /* fused assembly: 1 = 16 consecutive blocks per lane */
   uint32_t grid_ans, grid_indices, grid_wavelet, grid_assemble;
   uint32_t candidates;               /* shapes timed */
-  float stage_ms[5];                 /* median time of the chosen shape per stage */
+  float stage_ms[5];                 /* median time of the chosen shape per stage; NaN for a
+                                        stage the chosen shapes fold into another launch */
   float total_ms;
   uint32_t wavelet_fused;            /* 1: wavelet + index scan + DXT1 assembly in one kernel */
 } gst_tune_result;

@@ -90,46 +90,46 @@ void gsto_build_table(const uint16_t freqs[256], gsto_table_entry table[2048]) {
 /* ------------------------------------------------------------------------- */
 /* ans_decode                                                                 */
 
-/* ans/ans_decode.cl:108-157 for one 32-lane work-group.  The local-memory mask
- * protocol (:126-145) is a ballot: every lane sets its bit, the barrier makes all
+/* ans/ans_decode.cl:25-74 for one 32-lane work-group.  The local-memory mask
+ * protocol (:43-62) is a ballot: every lane sets its bit, the barrier makes all
  * bits visible, each lane reads the total and its prefix count, then clears.  The
  * sequential form below computes the mask first, then applies the reads. */
 int gsto_ans_decode_group(const gsto_table_entry table[2048], const uint8_t *stream,
                           size_t stream_len, uint32_t num_groups, uint32_t group,
                           uint8_t *out, uint32_t *final_state, uint32_t *final_word) {
   if (group >= num_groups || (uint64_t)num_groups * 4 > stream_len) return GSTO_ERR_STREAM;
-  const uint32_t offset = rd32(stream + 4 * group);                       /* :113 */
+  const uint32_t offset = rd32(stream + 4 * group);                       /* :30 */
   const uint32_t lo = group == 0 ? 4 * num_groups : rd32(stream + 4 * (group - 1));
   if ((offset & 3) || (lo & 3) || offset > stream_len || lo > offset ||
       offset - lo < 4 * GSTO_LANES)
     return GSTO_ERR_STREAM;
 
   uint32_t state[GSTO_LANES];
-  for (uint32_t j = 0; j < GSTO_LANES; ++j)                               /* :114 */
+  for (uint32_t j = 0; j < GSTO_LANES; ++j)                               /* :31 */
     state[j] = rd32(stream + offset - 4 * GSTO_LANES + 4 * j);
-  uint32_t next = (offset - GSTO_LANES * 4) / 2;                          /* :115 */
+  uint32_t next = (offset - GSTO_LANES * 4) / 2;                          /* :32 */
   const uint32_t lo_word = lo / 2;
 
   for (uint32_t i = 0; i < GSTO_SYMS_PER_LANE; ++i) {
     uint32_t mask = 0;
     uint8_t sym[GSTO_LANES];
     for (uint32_t j = 0; j < GSTO_LANES; ++j) {
-      const uint32_t slot = state[j] & (GSTO_TABLE_SIZE - 1);            /* :121 */
+      const uint32_t slot = state[j] & (GSTO_TABLE_SIZE - 1);            /* :38 */
       const gsto_table_entry *e = &table[slot];
-      state[j] = (state[j] >> GSTO_TABLE_LOG) * e->freq - e->cum_freq + slot;  /* :123 */
-      if (state[j] < GSTO_STATE_L) mask |= 1u << j;                      /* :127-129 */
+      state[j] = (state[j] >> GSTO_TABLE_LOG) * e->freq - e->cum_freq + slot;  /* :40-41 */
+      if (state[j] < GSTO_STATE_L) mask |= 1u << j;                      /* :44-46 */
       sym[j] = e->symbol;
     }
-    const uint32_t total = popc32(mask);                                  /* :134 */
+    const uint32_t total = popc32(mask);                                  /* :51 */
     for (uint32_t j = 0; j < GSTO_LANES; ++j) {
       if (!(mask >> j & 1)) continue;
-      const uint32_t skip = total - (popc32(mask & ((1u << j) - 1)) + 1);  /* :137-138 */
-      const uint32_t w = next - skip - 1;                                  /* :139 */
+      const uint32_t skip = total - (popc32(mask & ((1u << j) - 1)) + 1);  /* :54-55 */
+      const uint32_t w = next - skip - 1;                                  /* :56 */
       if (w < lo_word || w >= next) return GSTO_ERR_STREAM;
       state[j] = (state[j] << 16) | rd16(stream + 2 * (size_t)w);
     }
-    next -= total;                                                        /* :148 */
-    for (uint32_t j = 0; j < GSTO_LANES; ++j)                             /* :154-155 */
+    next -= total;                                                        /* :65 */
+    for (uint32_t j = 0; j < GSTO_LANES; ++j)                             /* :71-72 */
       out[j * GSTO_SYMS_PER_LANE + GSTO_SYMS_PER_LANE - 1 - i] = sym[j];
   }
   if (final_state) memcpy(final_state, state, sizeof(state));
@@ -249,7 +249,7 @@ void gsto_decode_indices_staged(const uint8_t *diffs, uint32_t nv, int32_t *out,
 /* ------------------------------------------------------------------------- */
 /* inv_wavelet                                                                */
 
-/* codec/inverse_wavelet.cl:98-100 */
+/* codec/inverse_wavelet.cl:14-16 */
 static int normalize_index(int idx, int range) {
   const int v = idx - (int)(idx >= range) * (idx - range + 2);
   return v < 0 ? -v : v;
@@ -257,7 +257,7 @@ static int normalize_index(int idx, int range) {
 
 enum { WDIM = 32 };
 
-/* codec/inverse_wavelet.cl:112-128: even output 2x of row y, stored transposed. */
+/* codec/inverse_wavelet.cl:28-44: even output 2x of row y, stored transposed. */
 static void wavelet_even(const int32_t *src, int32_t *dst, int x, int y, int len, int mid) {
   const int idx = 2 * x;
   const int prev = mid + normalize_index(idx - 1, len) / 2;
@@ -266,7 +266,7 @@ static void wavelet_even(const int32_t *src, int32_t *dst, int x, int y, int len
   dst[idx * WDIM + y] = src[y * WDIM + x] - (sp + sn + 2) / 4;
 }
 
-/* codec/inverse_wavelet.cl:130-148: odd output 2x+1 of row y, stored transposed. */
+/* codec/inverse_wavelet.cl:46-64: odd output 2x+1 of row y, stored transposed. */
 static void wavelet_odd(const int32_t *src, int32_t *dst, int x, int y, int len, int mid) {
   const int idx = mid + x;
   const int prev = normalize_index(2 * x, len);
@@ -275,7 +275,7 @@ static void wavelet_odd(const int32_t *src, int32_t *dst, int x, int y, int len,
   dst[(2 * x + 1) * WDIM + y] = src[y * WDIM + idx] + (dp + dn) / 2;
 }
 
-/* One level of codec/inverse_wavelet.cl:200-256: rows y < len over x < len, then
+/* One level of codec/inverse_wavelet.cl:116-172: rows y < len over x < len, then
  * columns, on a 32x32 int32 tile held in `src` with `scratch` as the transposed
  * buffer.  Each barrier phase writes words no other work-item of that phase reads,
  * so the phases are emulated as loops. */
@@ -293,9 +293,9 @@ void gsto_inv_wavelet_level(int32_t tile[1024], int len) {
   wavelet_level(tile, scratch, len);
 }
 
-/* codec/inverse_wavelet.cl:153-276, one 16x16 work-group over one tile: load with
- * the -128 bias (:177-188), levels len = 2, 4, 8, 16 (:200-234) and 32 (:236-256,
- * two rows per work-item, the same arithmetic), int8 store (:259-275). */
+/* codec/inverse_wavelet.cl:69-192, one 16x16 work-group over one tile: load with
+ * the -128 bias (:93-104), levels len = 2, 4, 8, 16 (:116-150) and 32 (:152-172,
+ * two rows per work-item, the same arithmetic), int8 store (:174-191). */
 void gsto_inv_wavelet_tile(const uint8_t in[1024], int8_t out[1024]) {
   int32_t local_data[2 * WDIM * WDIM];
   int32_t *src = local_data, *scratch = local_data + WDIM * WDIM;
@@ -307,7 +307,7 @@ void gsto_inv_wavelet_tile(const uint8_t in[1024], int8_t out[1024]) {
 /* ------------------------------------------------------------------------- */
 /* assemble                                                                   */
 
-/* codec/assemble.cl:315-338: YCoCg -> RGB565 with truncating division; the
+/* codec/assemble.cl:39-62: YCoCg -> RGB565 with truncating division; the
  * shifts are done on uint32 so negative channels keep their two's-complement bits
  * as the OpenCL int shifts do, and the ushort store keeps the low 16 bits. */
 uint16_t gsto_ycocg_to_rgb565(int y, int co, int cg) {
@@ -324,7 +324,7 @@ static uint32_t palette_word(const uint32_t *palette, uint32_t palette_words, in
   return palette[(uint32_t)idx];
 }
 
-/* codec/assemble.cl:340-357 over the (bx, by) grid of one image. */
+/* codec/assemble.cl:64-81 over the (bx, by) grid of one image. */
 static uint32_t assemble_dxt_rows(const int8_t *planes, const int32_t *indices,
                                   const uint32_t *palette, uint32_t palette_words,
                                   uint32_t nv, uint32_t t0, uint32_t t1, uint8_t *out) {
@@ -349,7 +349,7 @@ uint32_t gsto_assemble_dxt(const int8_t *planes, const int32_t *indices,
   return assemble_dxt_rows(planes, indices, palette, palette_words, nv, 0, nv, out);
 }
 
-/* codec/assemble.cl:359-406: 4-colour palette in 8-bit RGB, then 16 texels. */
+/* codec/assemble.cl:83-130: 4-colour palette in 8-bit RGB, then 16 texels. */
 static uint32_t assemble_rgb_rows(const int8_t *planes, const int32_t *indices,
                                   const uint32_t *palette, uint32_t palette_words,
                                   uint32_t width, uint32_t height, uint32_t t0, uint32_t t1,
@@ -470,8 +470,8 @@ static void ans_task(void *vctx, size_t b, size_t e) {
   }
 }
 
-/* Tile t of plane p: codec/inverse_wavelet.cl:163-167 (plane base and stride) and
- * :177-179, :259-275 (tile position). */
+/* Tile t of plane p: codec/inverse_wavelet.cl:79-83 (plane base and stride) and
+ * :93-95, :174-191 (tile position). */
 static void wavelet_task(void *vctx, size_t b, size_t e) {
   decode_ctx *c = (decode_ctx *)vctx;
   const uint32_t bx = c->hdr.width / 4, tiles_x = bx / WDIM;

@@ -83,7 +83,7 @@ int gsto_parse_header(const uint8_t *file, size_t len, gsto_header *hdr);
 /* Slot table from 256 uint16 frequencies (ans/build_table.cl:12-83). */
 void gsto_build_table(const uint16_t freqs[256], gsto_table_entry table[2048]);
 
-/* Decode one 32-stream rANS group (ans/ans_decode.cl:108-157).  `stream` is one
+/* Decode one 32-stream rANS group (ans/ans_decode.cl:25-74).  `stream` is one
  * ANS stream (u32 end offsets then groups), `stream_len` its size in bytes and
  * `num_groups` its group count.  Writes 8192 symbols in the reference's lane-major
  * order: lane j's 256 symbols at out[j*256 .. j*256+255].  Returns GSTO_OK, or
@@ -107,7 +107,7 @@ void gsto_decode_indices(const uint8_t *diffs, uint32_t nv, int32_t *out);
 void gsto_decode_indices_staged(const uint8_t *diffs, uint32_t nv, int32_t *out,
                                 int race_policy, int32_t garbage);
 
-/* Inverse 5-level LeGall 5/3 on one 32x32 tile (codec/inverse_wavelet.cl:98-276).
+/* Inverse 5-level LeGall 5/3 on one 32x32 tile (codec/inverse_wavelet.cl:14-192).
  * `in` is 1024 bytes (raster within the tile, +128 bias); `out` receives 1024 int8
  * in tile raster order. */
 void gsto_inv_wavelet_tile(const uint8_t in[1024], int8_t out[1024]);

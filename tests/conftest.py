@@ -11,6 +11,9 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 GOLDEN = os.path.join(ROOT, "tests", "golden")
 GOLDEN_TEXTURES = ("ref_test1", "ref_c0", "ref_small", "ref_wide")
+# Byte every GPU test output starts as (0xA5 blocks are valid DXT1, so a path that
+# writes nothing yields plausible bytes that only a comparison catches).
+POISON = 0xA5
 
 
 def pytest_configure(config):
@@ -42,7 +45,15 @@ class Device:
         return self.torch.cuda.current_stream().cuda_stream
 
     def empty(self, nbytes):
-        return self.torch.empty(max(int(nbytes), 1), dtype=self.torch.uint8, device=self.dev)
+        """Device bytes filled with POISON: a launch that writes nothing, or writes
+        short, cannot pass on what an earlier test left in a recycled block."""
+        t = self.torch.empty(max(int(nbytes), 1), dtype=self.torch.uint8, device=self.dev)
+        return self.poison(t)
+
+    def poison(self, t):
+        t.fill_(POISON)
+        self.torch.cuda.current_stream().synchronize()
+        return t
 
     def upload(self, data):
         host = self.torch.frombuffer(bytearray(bytes(data)), dtype=self.torch.uint8)

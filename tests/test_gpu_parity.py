@@ -132,6 +132,64 @@ def test_load_rgb_matches_oracle(device, name):
         assert hashlib.sha256(got).digest() == rgbs["mixed_rgb_sha256"].tobytes()
 
 
+def test_load_rgbs_uneven_batch_matches_oracle(device):
+    """LoadRGBs (codec/decoder.cpp:552-556) over images of uneven size in one staging
+    buffer: image i's 3*w*h RGB8 bytes follow image i-1's, each equal to the oracle and,
+    for ref_small, to the reference's own CPU decompression."""
+    rgbs = load_golden("ref_rgb")
+    files = [load_golden("ref_wide")["gtc"].tobytes(), load_golden("ref_small")["gtc"].tobytes(),
+             rgbs["mixed_gtc"].tobytes(), load_golden("ref_c0")["gtc"].tobytes()]
+    staging, hdrs = gst_amd.PackStaging(files)
+    assert len({(h.width, h.height) for h in hdrs}) > 1
+    d_cmp = device.upload(staging)
+    sizes = [3 * h.width * h.height for h in hdrs]
+    d_out = device.empty(sum(sizes) + 4096)  # a poisoned tail: nothing may be written past the images
+    gst_amd.LoadRGBs(hdrs, device.stream, d_cmp.data_ptr(), d_out.data_ptr())
+    got = device.download(d_out)
+    off = 0
+    for f, n in zip(files, sizes):
+        rc, expected = go.decode_rgb(f, nthreads=8)
+        assert rc == go.OK
+        assert got[off:off + n] == expected
+        off += n
+    assert got[off:] == bytes([0xA5]) * 4096
+    small = sizes[0]
+    assert got[small:small + sizes[1]] == rgbs["small_rgb"].tobytes()
+    assert gst_amd.ReadErrorBits(device.stream) == 0
+
+
+@pytest.mark.parametrize("name", ["ref_c0", "ref_small"])
+def test_stage_assemble_rgb_matches_oracle(device, name):
+    """The per-stage RGB8 assembly (gst_stage_assemble_rgb, codec/assemble.cl:83-129)
+    fed by the per-stage rANS, index and wavelet launches, into a poisoned output."""
+    gtc = load_golden(name)["gtc"].tobytes()
+    hdr = gst_amd.ParseHeader(gtc)
+    nv, P = hdr.num_blocks, hdr.palette_bytes
+    s = device.stream
+    d_file = device.upload(gtc)
+    base = d_file.data_ptr()
+    d_tables = device.empty(4 * 2048 * 4 + 16)
+    d_err = device.upload(b"\0\0\0\0")
+    d_sym = device.empty(7 * nv + P)
+    d_gs = device.empty(4 * (nv // 8192))
+    d_idx = device.empty(4 * nv)
+    d_planes = device.empty(6 * nv)
+    d_out = device.empty(3 * hdr.width * hdr.height)
+    gst_amd.StageBuildTable(base + 28, d_tables.data_ptr(), d_err.data_ptr(), s)
+    gst_amd.StageAnsDecode(hdr, base + 28 + 2048, d_tables.data_ptr(), d_sym.data_ptr(), d_gs.data_ptr(),
+                           d_err.data_ptr(), s)
+    gst_amd.StageDecodeIndices(hdr, d_sym.data_ptr(), d_gs.data_ptr(), d_idx.data_ptr(), s)
+    gst_amd.StageInvWavelet(hdr, d_sym.data_ptr(), d_planes.data_ptr(), s)
+    gst_amd.StageAssembleRGB(hdr, d_sym.data_ptr(), d_planes.data_ptr(), d_idx.data_ptr(), d_out.data_ptr(),
+                             d_err.data_ptr(), s)
+    rc, expected = go.decode_rgb(gtc)
+    assert rc == go.OK
+    assert device.download(d_out) == expected
+    if name == "ref_small":
+        assert device.download(d_out) == load_golden("ref_rgb")["small_rgb"].tobytes()
+    assert device.download(d_err) == b"\0\0\0\0"
+
+
 @pytest.mark.parametrize("w,h,seed", [(512, 256, 21), (768, 512, 22), (1024, 1024, 23), (2048, 1024, 24),
                                       (2048, 2048, 25), (4096, 4096, 1)])
 def test_encoder_textures_match_oracle(device, w, h, seed):

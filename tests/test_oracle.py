@@ -225,7 +225,7 @@ def test_ycocg_to_rgb565_inverts_encoder_transform_exhaustively():
 
 
 def test_rgb565_pack_keeps_low_bits_for_out_of_range_channels():
-    # (u16)(R << 11 | G << 5 | B) on int32 (codec/assemble.cl:324-338)
+    # (u16)(R << 11 | G << 5 | B) on int32 (codec/assemble.cl:48-62)
     for y, co, cg in ((0, 31, -63), (63, -31, 63), (-5, 17, 100), (127, -128, -128)):
         t = y - int(cg / 2)
         g = cg + t

@@ -40,8 +40,9 @@ def test_dry_run_completes_every_request_once():
 
 def test_admission_respects_budget_and_waits_for_release():
     files = _files()
-    need = max(_dev_bytes(f) for f in files)
-    # Room for exactly one request at a time; completions are held until released.
+    need = _dev_bytes(files[0])
+    # Room for exactly one request at a time (its input is returned when its decode
+    # completes, its output is held until released).
     with Server(need, dry_run=True, auto_release=False) as s:
         tickets = [s.submit(files[0]) for _ in range(3)]
         deadline = time.time() + 5
@@ -95,28 +96,54 @@ def test_batches_respect_image_and_block_limits():
 def test_poisson_load_generator_dry_run():
     files = _files()
     with Server(1 << 30, dry_run=True) as s:
-        lat_dev, lat_host, n, wall = s.run_load(files, qps=2000.0, duration_s=0.25, seed=3)
-        assert 300 <= n <= 800  # ~500 expected arrivals
-        assert np.all(lat_dev >= 0) and wall >= 0.2
+        lat_dev, lat_host, res = s.run_load(files, qps=2000.0, duration_s=0.25, seed=3)
+        assert 300 <= res["requests"] <= 800  # ~500 expected arrivals
+        assert np.all(lat_dev >= 0) and res["wall_s"] >= 0.2
+        assert res["failed"] == 0 and res["verified"] == 0
     with Server(1 << 30, dry_run=True) as s:
-        _, _, n, _ = s.run_load(files, closed_loop=8, duration_s=0.1, seed=3)
-        assert n >= 8
+        _, _, res = s.run_load(files, closed_loop=8, duration_s=0.1, seed=3)
+        assert res["requests"] >= 8
+
+
+def test_failed_launch_fails_its_batch_and_frees_its_memory():
+    """A batch whose launch fails part-way completes its requests with the error, frees
+    their memory only after the slot's stream has drained, and the server goes on: with
+    every second launch failing, the other batches still complete and the budget, which
+    holds one request at a time, keeps admitting."""
+    f = _files()[0]
+    need = _dev_bytes(f)
+    with Server(need, dry_run=True, fail_launch_every=2, slots=1) as s:
+        tickets = [s.submit(f, tag=i) for i in range(8)]
+        s.drain()
+        done = s.poll()
+        assert sorted(c.ticket for c in done) == sorted(tickets)
+        failed = [c for c in done if c.status != 0]
+        assert len(failed) == 4 and all(c.status == -5 for c in failed)  # GST_ERR_HIP
+        st = s.stats()
+        assert st["failed"] == 4 and st["completed"] == 8 and st["batches"] == 8
+        assert st["peak_arena_bytes"] <= need
+
+
+def test_prestaged_requests_take_no_copy_dry_run():
+    files = _files()
+    with Server(1 << 30, dry_run=True) as s:
+        for i, f in enumerate(files):
+            s.submit_request(f[:28], tag=i, d_file=0x1000 * (i + 1))  # never dereferenced in a dry run
+        s.submit(files[0], tag=99)
+        s.drain()
+        done = s.poll()
+        assert len(done) == len(files) + 1 and all(c.status == 0 for c in done)
+        st = s.stats()
+        assert st["prestaged"] == len(files) and st["submitted"] == len(files) + 1
+    with Server(1 << 30, dry_run=True) as s:
+        with pytest.raises(gst_amd.GstError) as e:
+            s.submit_request(files[0], d_file=0x1000, d_expected=0x1008)  # misaligned expected output
+        assert e.value.code == -1
 
 
 # ---- GPU ---------------------------------------------------------------------------------
 
-@pytest.mark.gpu
-@pytest.mark.parametrize("direct_launch", [False, True])
-def test_served_results_equal_alone_and_in_the_mix(device, direct_launch):
-    """Each request decoded alone (one at a time) and in a concurrent stream of
-    arrivals gives the same bit-exact blocks, with graph replays or stream launches."""
-    torch = device.torch
-    specs = [(501, 512, 512), (502, 1024, 1024), (503, 512, 256), (504, 2048, 2048), (505, 768, 512),
-             (506, 512, 1024)]
-    files, dxts = encode.make_textures(specs)
-    files += [load_golden("ref_test1")["gtc"].tobytes()]
-    dxts += [load_golden("ref_test1")["dxt"].tobytes()]
-
+def _hip_download():
     import ctypes
     hip = ctypes.CDLL("libamdhip64.so")
     hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
@@ -125,9 +152,33 @@ def test_served_results_equal_alone_and_in_the_mix(device, direct_launch):
         host = np.empty(int(c.output_bytes), np.uint8)
         assert hip.hipMemcpy(host.ctypes.data, c.d_output, int(c.output_bytes), 2) == 0
         return host.tobytes()
+    return download
 
+
+# Every size of the served mix (512^2 ... 4096^2) plus odd shapes.
+SERVED_SPECS = [(501, 512, 512), (502, 1024, 1024), (503, 512, 256), (504, 2048, 2048), (505, 768, 512),
+                (506, 512, 1024), (507, 4096, 4096)]
+
+
+@pytest.fixture(scope="module")
+def served_corpus():
+    files, dxts = encode.make_textures(SERVED_SPECS)
+    files += [load_golden("ref_test1")["gtc"].tobytes()]
+    dxts += [load_golden("ref_test1")["dxt"].tobytes()]
+    return files, dxts
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("direct_launch", [False, True])
+def test_served_results_equal_alone_and_in_the_mix(device, served_corpus, direct_launch):
+    """Each request decoded alone (one at a time) and in a concurrent stream of
+    arrivals gives the same bit-exact blocks, with graph replays or stream launches.
+    Outputs are poisoned before every decode, and the in-mix run also compares each
+    output with its expected blocks on the device."""
+    files, dxts = served_corpus
+    download = _hip_download()
     alone = {}
-    with Server(1 << 30, auto_release=False, direct_launch=direct_launch) as s:
+    with Server(1 << 30, auto_release=False, direct_launch=direct_launch, poison_outputs=True) as s:
         for i, f in enumerate(files):
             s.submit(f, tag=i)
             s.drain()
@@ -137,32 +188,114 @@ def test_served_results_equal_alone_and_in_the_mix(device, direct_launch):
             s.release(c.ticket)
     assert [alone[i] for i in range(len(files))] == dxts
 
-    with Server(1 << 30, auto_release=False, slots=3, direct_launch=direct_launch) as s:
+    d_expected = [device.upload(d) for d in dxts]
+    with Server(1 << 30, auto_release=False, slots=3, direct_launch=direct_launch, poison_outputs=True) as s:
         order = list(range(len(files))) * 4
         np.random.default_rng(0).shuffle(order)
         for i in order:
-            s.submit(files[i], tag=i)
+            s.submit_request(files[i], tag=i, d_expected=d_expected[i].data_ptr())
         s.drain()
         done = s.poll(4096)
         assert len(done) == len(order)
         assert max(c.batch_images for c in done) > 1, "arrivals were batched"
         for c in done:
             assert c.status == 0
+            assert c.verified == 1, "the device comparison found a difference (tag %d)" % c.tag
             assert download(c) == alone[c.tag]
+            s.release(c.ticket)
+        st = s.stats()
+        assert st["verified"] == len(order) and st["mismatches"] == 0
+        assert st["h2d_copies"] <= st["submitted"]
+
+
+@pytest.mark.gpu
+def test_device_comparison_reports_a_wrong_expected_output(device, served_corpus):
+    """The comparison is not vacuous: an expected output with one flipped byte is
+    reported as differing, the others as equal."""
+    files, dxts = served_corpus
+    wrong = bytearray(dxts[0])
+    wrong[len(wrong) // 2] ^= 0x10
+    d_good = device.upload(dxts[0])
+    d_wrong = device.upload(bytes(wrong))
+    with Server(1 << 30, poison_outputs=True) as s:
+        s.submit_request(files[0], tag=0, d_expected=d_good.data_ptr())
+        s.submit_request(files[0], tag=1, d_expected=d_wrong.data_ptr())
+        s.submit(files[0], tag=2)
+        s.drain()
+        done = {c.tag: c for c in s.poll()}
+    assert done[0].verified == 1 and done[1].verified == 2 and done[2].verified == 0
+    assert all(c.status == 0 for c in done.values())
+
+
+@pytest.mark.gpu
+def test_prestaged_inputs_decode_without_copies(device, served_corpus):
+    """Files already in HBM are decoded in place: no host-to-device copy, same blocks."""
+    files, dxts = served_corpus
+    d_files = [device.upload(f) for f in files]
+    d_expected = [device.upload(d) for d in dxts]
+    download = _hip_download()
+    with Server(1 << 30, auto_release=False, poison_outputs=True) as s:
+        for i, (f, df) in enumerate(zip(files, d_files)):
+            s.submit_request(f[:28], tag=i, d_file=df.data_ptr(), d_expected=d_expected[i].data_ptr())
+        s.drain()
+        done = s.poll()
+        assert len(done) == len(files)
+        for c in done:
+            assert c.status == 0 and c.verified == 1
+            assert download(c) == dxts[c.tag]
+            s.release(c.ticket)
+        st = s.stats()
+        assert st["prestaged"] == len(files) and st["h2d_bytes"] == 0
+
+
+@pytest.mark.gpu
+def test_served_poisson_load_checks_every_completion(device, served_corpus):
+    """The load generator under Poisson arrivals and in a closed loop, host-submitted
+    and pre-staged: every completion is compared with its expected blocks on the device."""
+    files, dxts = served_corpus
+    d_files = [device.upload(f) for f in files]
+    d_expected = [device.upload(d) for d in dxts]
+    exp = [t.data_ptr() for t in d_expected]
+    with Server(1 << 30, poison_outputs=True, copy_to_host=True) as s:
+        _, _, res = s.run_load(files, qps=2000.0, duration_s=0.3, seed=11, pool_expected=exp)
+        assert res["requests"] > 100 and res["failed"] == 0
+        assert res["verified"] == res["requests"] and res["mismatches"] == 0
+    with Server(1 << 30, poison_outputs=True) as s:
+        _, _, res = s.run_load(files, closed_loop=32, duration_s=0.3, seed=12, pool_expected=exp,
+                               pool_device=[t.data_ptr() for t in d_files])
+        assert res["requests"] > 32 and res["failed"] == 0
+        assert res["verified"] == res["requests"] and res["mismatches"] == 0
+
+
+@pytest.mark.gpu
+def test_plain_mode_matches(device, served_corpus):
+    files, dxts = served_corpus
+    g = load_golden("ref_wide")
+    download = _hip_download()
+    d_expected = device.upload(dxts[-2])
+    with Server(1 << 28, plain=True, auto_release=False, poison_outputs=True) as s:
+        s.submit(g["gtc"].tobytes(), tag=0)
+        s.submit_request(files[-2], tag=1, d_expected=d_expected.data_ptr())  # 4096x4096
+        s.drain()
+        done = {c.tag: c for c in s.poll()}
+        assert done[0].status == 0 and download(done[0]) == g["dxt"].tobytes()
+        assert done[1].status == 0 and done[1].verified == 1 and download(done[1]) == dxts[-2]
+        for c in done.values():
             s.release(c.ticket)
 
 
 @pytest.mark.gpu
-def test_plain_mode_matches(device):
-    import ctypes
-    hip = ctypes.CDLL("libamdhip64.so")
-    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
-    g = load_golden("ref_wide")
-    with Server(1 << 28, plain=True, auto_release=False) as s:
-        s.submit(g["gtc"].tobytes())
+def test_plain_mode_reports_each_corrupt_request(device):
+    """Plain mode gives every request its own device error bits: a corrupt file fails,
+    the valid requests before and after it do not."""
+    import struct
+    gtc = load_golden("ref_c0")["gtc"].tobytes()
+    bad = bytearray(gtc)
+    struct.pack_into("<H", bad, 28, struct.unpack_from("<H", bad, 28)[0] + 3)  # Y frequencies no longer sum to 2048
+    with Server(1 << 28, plain=True) as s:
+        s.submit(gtc, tag=0)
+        s.submit(bytes(bad), tag=1)
+        s.submit(gtc, tag=2)
         s.drain()
-        (c,) = s.poll()
-        host = np.empty(int(c.output_bytes), np.uint8)
-        assert hip.hipMemcpy(host.ctypes.data, c.d_output, int(c.output_bytes), 2) == 0
-        assert host.tobytes() == g["dxt"].tobytes()
-        s.release(c.ticket)
+        done = {c.tag: c.status for c in s.poll()}
+    assert done == {0: 0, 1: -3, 2: 0}
