@@ -414,6 +414,61 @@ def run_served(gpu, qps_list, duration_s, budget_gib=32, closed_loop=512, tuned=
     return out
 
 
+def run_stage_kernels(gpu, reps=20):
+    """The reference's five stage launches one at a time (codec/decoder.cpp:165-417,
+    gst_stage_* here) on the single-image configurations: C1 (the 512x512 seed-0
+    texture) and C2 (the 4096x4096 seed-1 texture, noise 0.3 of range).  Each stage is
+    timed with hipEvents on the decode stream (median of `reps`), its read+write bytes
+    from the per-stage table, and the final DXT1 blocks are checked bit-exact."""
+    import gst_amd
+    from gst_amd import encode
+    torch = gpu.torch
+    out = {}
+    for name, (seed, w, h) in (("C1_512x512", (0, 512, 512)), ("C2_4096x4096", (1, 4096, 4096))):
+        (f,), (d,) = encode.make_textures([(seed, w, h)], encoder="gentenc")
+        hdr = gst_amd.ParseHeader(f)
+        nv, P, S = hdr.num_blocks, hdr.palette_bytes, hdr.stream_bytes
+        d_file, (base,) = gpu.upload_many([f])
+        t_tab, t_err, t_sym, t_gs, t_idx, t_pl, t_out = (gpu.empty(n) for n in (
+            4 * 2048 * 4 + 16, 256, 7 * nv + P, 4 * (nv // 8192), 4 * nv, 6 * nv, 8 * nv))
+        t_err.zero_()
+        st = gpu.stream
+        calls = {
+            "build_table": (lambda: gst_amd.StageBuildTable(base + 28, t_tab.data_ptr(), t_err.data_ptr(), st),
+                            2048 + 32768),
+            "ans_decode": (lambda: gst_amd.StageAnsDecode(hdr, base + 28 + 2048, t_tab.data_ptr(), t_sym.data_ptr(),
+                                                          t_gs.data_ptr(), t_err.data_ptr(), st),
+                           S + 32768 + 7 * nv + P + 4 * (nv // 8192)),
+            "decode_indices": (lambda: gst_amd.StageDecodeIndices(hdr, t_sym.data_ptr(), t_gs.data_ptr(),
+                                                                  t_idx.data_ptr(), st), nv + 4 * nv),
+            "inv_wavelet": (lambda: gst_amd.StageInvWavelet(hdr, t_sym.data_ptr(), t_pl.data_ptr(), st),
+                            6 * nv + 6 * nv),
+            "assemble": (lambda: gst_amd.StageAssembleDXT(hdr, t_sym.data_ptr(), t_pl.data_ptr(), t_idx.data_ptr(),
+                                                          t_out.data_ptr(), t_err.data_ptr(), st),
+                         6 * nv + 4 * nv + P + 8 * nv),
+        }
+        for fn, _ in calls.values():  # one pass in pipeline order leaves every stage's inputs
+            fn()
+        torch.cuda.synchronize()
+        if t_out.cpu().numpy().tobytes() != d or t_err.cpu().numpy()[:4].any():
+            raise SystemExit("stage kernels, %s: output differs from the expected DXT1 blocks" % name)
+        res = {}
+        for k, (fn, nbytes) in calls.items():
+            ev = [torch.cuda.Event(enable_timing=True) for _ in range(reps + 1)]
+            fn()
+            ev[0].record()
+            for i in range(reps):
+                fn()
+                ev[i + 1].record()
+            torch.cuda.synchronize()
+            ms = float(np.median([ev[i].elapsed_time(ev[i + 1]) for i in range(reps)]))
+            res[k] = {"us": round(ms * 1e3, 2), "bytes": int(nbytes), "GB_s": round(nbytes / (ms * 1e-3) / 1e9, 1),
+                      "hbm_frac": round(nbytes / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4)}
+        out[name] = res
+        log("stage kernels %s: %s" % (name, res))
+    return out
+
+
 def run_ans_api(files, reps=3):
     """The stand-alone rANS API (gst_ans_*, the OpenCLDecoder counterpart) over the
     corpus' luma streams: per image, RebuildTable from its Y frequencies, then one Decode
@@ -465,6 +520,9 @@ def main():
     ap.add_argument("--served-seconds", type=float, default=3.0)
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes (no autotune)")
     ap.add_argument("--no-ans-api", action="store_true", help="skip the stand-alone rANS API leg")
+    ap.add_argument("--stage-kernels", action="store_true",
+                    help="only time the five per-stage kernels (gst_stage_*) on the 512x512 and 4096x4096 "
+                         "single-image configurations and print their GB/s")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device-to-device copy of 512 MiB (5x), the reference kernel for "
                          "scaling FETCH_SIZE / WRITE_SIZE counters to bytes")
@@ -488,6 +546,9 @@ def main():
     from gst_amd import encode
     if not gst_amd.InitializeDecoder(0):
         raise SystemExit("cuda:0 is not a usable gfx950")
+    if args.stage_kernels:
+        print(json.dumps({"stage_kernels": run_stage_kernels(gpu)}), flush=True)
+        return
 
     t0 = time.perf_counter()
     specs = corpus_c3(args.images)

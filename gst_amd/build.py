@@ -86,6 +86,39 @@ def build_cpp_tests():
     return outs
 
 
+def build_sanitizer_tests():
+    """Host-sanitizer builds of the host code for tests/test_sanitizers.py (CPU only):
+    the server's dry-run driver with ThreadSanitizer and with AddressSanitizer (the host
+    side of libgst_amd's sources; device code is built as usual and never run by it), the
+    encoder's C++ test and the oracle with AddressSanitizer + UndefinedBehaviorSanitizer,
+    and the oracle's thread pool with ThreadSanitizer."""
+    cpp = os.path.join(ROOT, "tests", "cpp")
+    lib_srcs = [os.path.join(CSRC, n) for n in ("gst_serve.cpp", "gst_decoder.cpp", "gst_kernels.hip", "gst_ans.cpp")]
+    outs = []
+    for kind in ("thread", "address"):
+        out = os.path.join(cpp, "serve_dryrun_%s" % ("tsan" if kind == "thread" else "asan"))
+        outs.append(_atomic_build(out, lambda o, kind=kind: [
+            HIPCC, "--offload-arch=gfx950", "-O1", "-g", "-std=c++17", "-Wno-unused-command-line-argument",
+            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+            "-Xarch_host", "-fsanitize=%s" % kind, "-Xarch_host", "-fno-omit-frame-pointer",
+            "-I" + INCLUDE, "-I" + CSRC, *lib_srcs, os.path.join(cpp, "serve_dryrun_test.cpp"), "-o", o,
+            "-lpthread"]))
+    gcc = shutil.which("gcc") or "gcc"
+    cxx = shutil.which("g++") or "g++"
+    oracle = os.path.join(ROOT, "oracle")
+    for tag, flags in (("asan", ["-fsanitize=address,undefined", "-fno-sanitize-recover=undefined"]),
+                       ("tsan", ["-fsanitize=thread"])):
+        outs.append(_atomic_build(os.path.join(cpp, "oracle_sanitize_%s" % tag), lambda o, flags=flags: [
+            gcc, "-std=c11", "-O1", "-g", "-fno-omit-frame-pointer", *flags, "-pthread", "-I" + oracle,
+            os.path.join(oracle, "gst_oracle.c"), os.path.join(cpp, "oracle_sanitize_test.c"), "-o", o]))
+    outs.append(_atomic_build(os.path.join(cpp, "encoder_asan"), lambda o: [
+        cxx, "-std=c++17", "-O1", "-g", "-fno-omit-frame-pointer", "-fsanitize=address,undefined",
+        "-fno-sanitize-recover=undefined", "-pthread", "-ffp-contract=off", "-I" + INCLUDE, "-I" + CSRC,
+        os.path.join(cpp, "encoder_test.cpp"), os.path.join(CSRC, "gst_encode.cpp"),
+        os.path.join(CSRC, "gst_gentc.cpp"), "-o", o]))
+    return outs
+
+
 def build_oracle():
     """The checker libraries (not part of the product)."""
     oracle = os.path.join(ROOT, "oracle")
@@ -99,6 +132,7 @@ def build_all():
     build_encoder()
     build_cpp_tests()
     build_oracle()
+    build_sanitizer_tests()
 
 
 if __name__ == "__main__":
