@@ -31,6 +31,7 @@ STATUS_NAMES = {
     -5: "GST_ERR_HIP",
     -6: "GST_ERR_NOT_INITIALIZED",
     -7: "GST_ERR_BUSY",
+    -8: "GST_ERR_TOO_LARGE",
 }
 DEVERR_STREAM_OFFSETS = 0x1
 DEVERR_FREQUENCIES = 0x2
@@ -123,6 +124,7 @@ def _declare(lib):
         "gst_decompress_dxt": ([ctypes.c_char_p, _SZ, _P, _SZ], ctypes.c_int),
         "gst_load_compressed_dxt": ([_HP, _P, _P, _P, _U32, _P, _P], ctypes.c_int),
         "gst_load_compressed_dxts": ([_HP, _U32, _P, _P, _P, _U32, _P, _P], ctypes.c_int),
+        "gst_load_compressed_dxt_noalloc": ([_HP, _P, _P, _P, _P], ctypes.c_int),
         "gst_read_error_bits_async": ([_P, _P], ctypes.c_int),
         "gst_load_rgb": ([_HP, _P, _P, _P, _U32, _P, _P], ctypes.c_int),
         "gst_load_rgbs": ([_HP, _U32, _P, _P, _P, _U32, _P, _P], ctypes.c_int),
@@ -244,6 +246,13 @@ def LoadCompressedDXT(hdr, stream, d_cmp, d_out, init=(), want_event=False):
 def LoadCompressedDXTs(hdrs, stream, d_cmp, d_out, init=(), want_event=False):
     return _load(lib().gst_load_compressed_dxts, "LoadCompressedDXTs", hdrs, stream, d_cmp, d_out, init,
                  want_event)
+
+
+def LoadCompressedDXTNoAlloc(hdr, stream, d_cmp, d_out, d_err_bits):
+    """gst_load_compressed_dxt_noalloc: one image, no allocation (static scratch);
+    error bits are OR-ed into the device word at d_err_bits."""
+    _check(lib().gst_load_compressed_dxt_noalloc(ctypes.byref(hdr), stream, d_cmp, d_out, d_err_bits),
+           "LoadCompressedDXTNoAlloc")
 
 
 def LoadRGB(hdr, stream, d_cmp, d_out, init=(), want_event=False):

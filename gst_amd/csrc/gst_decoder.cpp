@@ -411,6 +411,8 @@ int gst_initialize_decoder(int device) {
   // The kernels need 256-thread work-groups and 16 KiB + 8.4 KiB of LDS
   // (codec/decoder.cpp:558-591 checks the work-group sizes of the reference's).
   if (prop.maxThreadsPerBlock < 512 || prop.sharedMemPerBlock < 32768) return GST_ERR_NOT_INITIALIZED;
+  // The no-allocation entry point's static scratch on this device (current here).
+  GST_HIP(gst::ResolveNoAllocScratch(device));
   g.device = device;
   g.cu_count = prop.multiProcessorCount;
   g.initialized = true;
@@ -534,6 +536,46 @@ int gst_load_compressed_dxt(const gst_header *hdr, gst_stream stream, const void
                             uint32_t num_init, const gst_event *init, gst_event *done) {
   return DecodeStaging(hdr, 1, static_cast<hipStream_t>(stream), d_cmp, d_out, false, num_init, init,
                        done);
+}
+
+int gst_load_compressed_dxt_noalloc(const gst_header *hdr, gst_stream stream, const void *d_cmp, void *d_out,
+                                    uint32_t *d_err_bits) {
+  if (!hdr || !d_cmp || !d_out || !d_err_bits) return GST_ERR_INVALID_ARGUMENT;
+  int rc = EnsureInit();
+  if (rc != GST_OK) return rc;
+  if ((rc = CheckHeader(*hdr)) != GST_OK) return rc;
+  int device = 0;
+  GST_HIP(hipGetDevice(&device));
+  uint8_t *base = gst::NoAllocScratch(device);
+  if (!base) {
+    // First call on a device the library was not initialized on.
+    GST_HIP(gst::ResolveNoAllocScratch(device));
+    base = gst::NoAllocScratch(device);
+  }
+  // The decode's two kernels (rANS with tables built in LDS, then wavelet + index scan +
+  // DXT1 assembly) read and write only the symbols, the index stream's partial sums and
+  // its group totals: [symbols 7nv+P][partial sums nv/64 + nv/8][group totals nv/2048].
+  const size_t nv = NumBlocks(*hdr), P = hdr->palette_bytes;
+  Carve c;
+  size_t off = 0;
+  c.symbols = off; off += AlignUp(7 * nv + P, kAlign);
+  c.indices = off; off += AlignUp(4 * (nv / gst::kSymsPerLane + nv / gst::kRowBlocks), kAlign);
+  c.group_sums = off; off += AlignUp(4 * (nv / gst::kGroupSyms), kAlign);
+  c.planes = 0;  // not read: the wavelet runs inside the assembly
+  c.total = off;
+  if (c.total > gst::kNoAllocScratchBytes) return GST_ERR_TOO_LARGE;
+  const StageConfig cfg = DefaultStageConfig(false);
+  const uint8_t *cmp = static_cast<const uint8_t *>(d_cmp);
+  const size_t offsets_bytes = AlignUp(32, 512);  // codec/decoder.cpp:109-111 for one image
+  KernelArgs a;
+  memset(&a, 0, sizeof(a));
+  FillDesc(*hdr, cmp + offsets_bytes, cmp + offsets_bytes + 2048, base, c, 0, static_cast<uint8_t *>(d_out),
+           false, &a.single);
+  a.err = d_err_bits;
+  uint32_t grids[gst::kNumStages];
+  for (uint32_t st = 0; st < gst::kNumStages; ++st)
+    grids[st] = gst::SinglePassGrid(st, gst::StageUnitsOf(a.single, st, GroupsPerUnit(cfg)), cfg);
+  return LaunchAll(a, grids, cfg, static_cast<hipStream_t>(stream));
 }
 
 int gst_load_compressed_dxts(const gst_header *hdrs, uint32_t n, gst_stream stream, const void *d_cmp,

@@ -192,6 +192,14 @@ struct CompareArgs {
 };
 hipError_t LaunchCompare(const CompareArgs &a, hipStream_t stream);
 
+// Static device scratch of gst_load_compressed_dxt_noalloc: one array per GPU, held
+// until the process ends.  ResolveNoAllocScratch looks up the array's address on the
+// current device (once per device; the caller makes `device` current);
+// NoAllocScratch returns it, or null before it was resolved.
+constexpr size_t kNoAllocScratchBytes = size_t{12} << 20;
+hipError_t ResolveNoAllocScratch(int device);
+uint8_t *NoAllocScratch(int device);
+
 // The decoder library's one-time device set-up (gst_initialize_decoder(0) on first use).
 int EnsureDecoderInit();
 

@@ -5,6 +5,8 @@
 // single image passed by value.  Reference semantics are cited per kernel.
 #include "gst_internal.h"
 
+#include <mutex>
+
 namespace gst {
 namespace {
 
@@ -1593,6 +1595,37 @@ hipError_t LaunchCompare(const CompareArgs &a, hipStream_t stream) {
   if (a.n > kCompareJobs) return hipErrorInvalidValue;
   hipLaunchKernelGGL(CompareKernel, dim3(64, a.n), dim3(256), 0, stream, a);
   return hipGetLastError();
+}
+
+namespace {
+
+// The no-allocation entry point's scratch.  In host code `scratch` names a host-side
+// stand-in, not device memory: host code uses only the address ResolveNoAllocScratch
+// obtained for each device.
+__device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint8_t *g_scratch_ptr[kMaxDevices] = {};
+
+}  // namespace
+
+hipError_t ResolveNoAllocScratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return hipSuccess;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  if (!p) return hipErrorInvalidSymbol;
+  g_scratch_ptr[device] = static_cast<uint8_t *>(p);
+  return hipSuccess;
+}
+
+uint8_t *NoAllocScratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return g_scratch_ptr[device];
 }
 
 hipError_t LaunchAnsGroups(const uint32_t *table, const uint8_t *data, uint32_t data_len, uint32_t groups,

@@ -36,6 +36,7 @@ extern "C" {
 #define GST_ERR_HIP (-5)
 #define GST_ERR_NOT_INITIALIZED (-6)
 #define GST_ERR_BUSY (-7)
+#define GST_ERR_TOO_LARGE (-8)
 
 /* Device error bits raised by the kernels (gst_read_error_bits). */
 #define GST_DEVERR_STREAM_OFFSETS 0x1u  /* group end offset malformed, or a renormalisation
@@ -82,6 +83,21 @@ int gst_decompress_dxt(const uint8_t *file, size_t len, uint8_t *out, size_t out
 int gst_load_compressed_dxt(const gst_header *hdr, gst_stream stream, const void *d_cmp,
                             void *d_out, uint32_t num_init, const gst_event *init,
                             gst_event *done);
+
+/* No counterpart in the reference: LoadCompressedDXT for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The same single-image decode as gst_load_compressed_dxt (d_cmp in
+ * the reference staging layout, d_out w*h/2 bytes), with no allocation of any kind: its
+ * scratch is a static device array of GST_NOALLOC_SCRATCH_BYTES that each GPU the library
+ * runs on holds until the process ends, enough for images of up to 2^20 blocks (e.g.
+ * 4096x4096); larger images return GST_ERR_TOO_LARGE.  Device error bits (GST_DEVERR_*)
+ * are OR-ed into *d_err_bits, a device word the caller owns and clears.  Every call on
+ * one GPU shares that GPU's array: calls that overlap on one GPU (other streams, other
+ * threads) race and must be ordered by the caller.  No init events and no returned event:
+ * record one on `stream` after the call. */
+#define GST_NOALLOC_SCRATCH_BYTES (12u << 20)
+int gst_load_compressed_dxt_noalloc(const gst_header *hdr, gst_stream stream, const void *d_cmp,
+                                    void *d_out, uint32_t *d_err_bits);
 
 /* Replaces GenTC::LoadCompressedDXTs (codec/decoder.h:23-25, codec/decoder.cpp:540-544):
  * n images in one staging buffer, outputs back to back.  Unlike the reference the
