@@ -756,8 +756,6 @@ int gst_batch_create(const gst_batch_config *cfg, gst_batch **out) {
   b->stage.ans_waves = cfg->ans_waves_per_group ? cfg->ans_waves_per_group : 4;
   b->stage.wavelet_waves = cfg->wavelet_waves_per_group ? cfg->wavelet_waves_per_group : 1;
   b->stage.ans_build_tables = (cfg->separate_table_stage || b->stage.ans_waves < 4) ? 0 : 1;
-  b->stage.ans_wide_table =
-      (cfg->ans_wide_table && b->stage.ans_build_tables && b->stage.ans_groups_per_wave == 2) ? 1 : 0;
   b->stage.fused_waves = (cfg->rgb_output || cfg->separate_index_scan)
                              ? 0
                              : (cfg->assembly_waves_per_group ? cfg->assembly_waves_per_group : 4);
@@ -858,16 +856,12 @@ int gst_batch_autotune(gst_batch *b, gst_stream stream, uint32_t reps, gst_tune_
   static const uint32_t kAnsShapes[][2] = {{1, 2}, {1, 4}, {1, 8}, {2, 1}, {2, 2}, {2, 4}, {2, 8}};
   float best_ms = 1e30f, best_ans = 0.f;
   for (const auto &shape : kAnsShapes) {
-    // build: 0 = tables from the table stage, 1 = built in LDS, 2 = built in LDS as
-    // 8-byte entries (two groups a wave only).
-    for (uint32_t build = 0; build < 3; ++build) {
+    for (uint32_t build = 0; build < 2; ++build) {
       if (build && shape[1] < 4) continue;  // the LDS build needs 256 threads
-      if (build == 2 && shape[0] != 2) continue;
       StageConfig c = best;
       c.ans_groups_per_wave = shape[0];
       c.ans_waves = shape[1];
-      c.ans_build_tables = build ? 1 : 0;
-      c.ans_wide_table = build == 2 ? 1 : 0;
+      c.ans_build_tables = build;
       b->stage = c;
       if ((rc = UploadPlan(b, s, units)) != GST_OK) return rc;
       const uint32_t cand[] = {cus, 2 * cus, 4 * cus, 8 * cus, units[gst::kStageAns]};
@@ -880,8 +874,7 @@ int gst_batch_autotune(gst_batch *b, gst_stream stream, uint32_t reps, gst_tune_
           best_ans = ms;
           best.ans_groups_per_wave = c.ans_groups_per_wave;
           best.ans_waves = c.ans_waves;
-          best.ans_build_tables = c.ans_build_tables;
-          best.ans_wide_table = c.ans_wide_table;
+          best.ans_build_tables = build;
           grids[gst::kStageAns] = g;
         }
       }
@@ -1013,7 +1006,6 @@ int gst_batch_autotune(gst_batch *b, gst_stream stream, uint32_t reps, gst_tune_
   res.wavelet_waves_per_group = best.wavelet_waves;
   res.assembly_waves_per_group = best.fused_waves;
   res.ans_builds_tables = best.ans_build_tables;
-  res.ans_wide_table = best.ans_wide_table;
   res.assembly_wide = best.fused_wide;
   res.wavelet_fused = best.tile_fused;
   res.grid_ans = grids[gst::kStageAns];

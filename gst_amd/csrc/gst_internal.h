@@ -137,8 +137,6 @@ struct StageConfig {
                                  // assembly (two tile positions per 3-wave work-group);
                                  // the wavelet and index stages launch nothing
   uint32_t rgb;                  // 1: RGB8 assembly
-  uint32_t ans_wide_table;       // 1 (with ans_build_tables, 2 groups a wave): 8-byte table
-                                 // entries in LDS
 };
 
 // Work-groups that give every unit of `units` its own pass (no grid-stride repeat).

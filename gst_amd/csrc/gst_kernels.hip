@@ -329,7 +329,7 @@ __device__ __forceinline__ void TouchStreamTail(__amdgpu_buffer_rsrc_t rs, uint3
   }
 }
 
-template <bool kSum, uint32_t GPW, bool kWide = false>
+template <bool kSum, uint32_t GPW>
 __device__ __forceinline__ void DecodeHalfGroups(const uint32_t *tab, uint16_t *ring_wave, uint16_t *ring_half,
                                                  uint32_t *sink, __amdgpu_buffer_rsrc_t rs, AnsRing ring,
                                                  uint32_t lane,
@@ -340,7 +340,11 @@ __device__ __forceinline__ void DecodeHalfGroups(const uint32_t *tab, uint16_t *
   const uint32_t j = lane & 31;
   uint32_t upper = lane < 32 ? 0u : 0xFFFFFFFFu;
   asm volatile("" : "+v"(upper));  // keep the half-wave mask an opaque VGPR (one v_and below)
-  const uint64_t active_mask = __builtin_amdgcn_ballot_w64(active);
+  // Renormalisation threshold per lane: L for a lane with a stream, 0 (never) for one
+  // without, so the compare's mask is already the renormalising lanes and neither the
+  // ballot nor the select needs a scalar AND with the active lanes.
+  uint32_t thr = active ? kStateL : 0u;
+  asm volatile("" : "+v"(thr));
   uint16_t *const ringA = ring_wave;
   uint16_t *const ringB = ring_wave + kRingWords;
   uint32_t topA = ring.topA, topB = ring.topB;
@@ -391,19 +395,11 @@ __device__ __forceinline__ void DecodeHalfGroups(const uint32_t *tab, uint16_t *
     for (uint32_t blk = 0; blk < 64 / kRefillSteps; ++blk) {
 #pragma unroll
       for (uint32_t s = 0; s < kRefillSteps; ++s) {
-        // One LDS read per step: e = f | (slot - cum) << 12 | symbol << 24, or with
-        // kWide the pair {f | symbol << 24, slot - cum}, whose first word v_mad_u32_u24
-        // takes as it is (it reads the low 24 bits) and whose second is the addend.
-        uint32_t fs;
-        if constexpr (kWide) {
-          const uint2 e = reinterpret_cast<const uint2 *>(tab)[x & (kTableSize - 1)];
-          fs = e.x;
-          x = __umul24(x >> kTableLog, e.x) + e.y;
-        } else {
-          fs = tab[x & (kTableSize - 1)];
-          x = __umul24(x >> kTableLog, fs & 0xFFFu) + __builtin_amdgcn_ubfe(fs, 12, 11);
-        }
-        const uint64_t m = __builtin_amdgcn_uicmp(x, kStateL, 36 /* ULT */) & active_mask;
+        // One LDS read per step: e = f | (slot - cum) << 12 | symbol << 24.
+        const uint32_t fs = tab[x & (kTableSize - 1)];
+        x = __umul24(x >> kTableLog, fs & 0xFFFu) + __builtin_amdgcn_ubfe(fs, 12, 11);
+        const bool need = x < thr;
+        const uint64_t m = __builtin_amdgcn_uicmp(x, thr, 36 /* ULT */);
         const uint32_t mlo = static_cast<uint32_t>(m), mhi = static_cast<uint32_t>(m >> 32);
         const uint32_t tlo = __popc(mlo), thi = __popc(mhi);
         pA -= tlo;
@@ -418,7 +414,7 @@ __device__ __forceinline__ void DecodeHalfGroups(const uint32_t *tab, uint16_t *
           const uint32_t rank = __builtin_amdgcn_mbcnt_hi(mhi, __builtin_amdgcn_mbcnt_lo(mlo, 0u));
           const uint32_t widx = rank + pA + (upper & (pB - tlo - pA));
           const uint32_t xn = (x << 16) | ring_half[widx & (kRingWords - 1)];
-          x = (active && x < kStateL) ? xn : x;
+          x = need ? xn : x;
         }
         // Step 8*blk + s lands at byte 63 - 8*blk - s of the sector.  The symbol (byte
         // 3 of fs) goes in with v_perm: selector codes 4-7 take bytes of the first
@@ -544,20 +540,13 @@ __device__ __forceinline__ void LdsBarrier() { asm volatile("s_waitcnt lgkmcnt(0
 // Every LDS access of A and C is one word per lane at consecutive addresses.  A table
 // that does not sum to 2048 becomes the identity table.  Returns whether the table is
 // valid; ends with a work-group barrier.
-//
-// With kWide the entries are the 8-byte pairs {f | x << 24, slot - cum} in `tab`
-// (2 x 2048 words) and the marks live in `marks` (2048 words of the work-group's ring
-// space, free until the table is built); otherwise the marks are the 4-byte entries'
-// own words.
-template <uint32_t THREADS, bool kWide = false>
-__device__ __forceinline__ bool BuildTableInLds(uint32_t f, uint32_t *tab, uint32_t *marks, uint32_t *fc,
-                                                uint8_t *sym_of_rank, uint32_t *wsum, uint32_t *before) {
+template <uint32_t THREADS>
+__device__ __forceinline__ bool BuildTableInLds(uint32_t f, uint32_t *tab, uint32_t *fc, uint8_t *sym_of_rank,
+                                                uint32_t *wsum, uint32_t *before) {
   constexpr uint32_t kWaves = THREADS / 64, kSlotsPerWave = kTableSize / kWaves, kChunks = kSlotsPerWave / 64;
   static_assert(THREADS >= 256 && kWaves <= 8 && kChunks * 64 * kWaves == kTableSize, "launch shape");
   const uint32_t t = threadIdx.x, lane = t & 63, wave = Uniform(t >> 6);
-  if constexpr (!kWide) marks = tab;
-  uint32_t *const mine = marks + wave * kSlotsPerWave + lane;
-  uint2 *const mine2 = reinterpret_cast<uint2 *>(tab) + wave * kSlotsPerWave + lane;
+  uint32_t *const mine = tab + wave * kSlotsPerWave + lane;
 #pragma unroll
   for (uint32_t j = 0; j < kChunks; ++j) mine[64 * j] = 0;
   if (t < kWaves) before[t] = 0;
@@ -581,7 +570,7 @@ __device__ __forceinline__ bool BuildTableInLds(uint32_t f, uint32_t *tab, uint3
     fc[t] = f | (cum << 12);
     if (f != 0) {  // the table sums to 2048, so cum + f <= 2048: the mark is inside it
       sym_of_rank[excl >> 24] = static_cast<uint8_t>(t);
-      marks[cum] = 1;
+      tab[cum] = 1;
     }
 #pragma unroll
     for (uint32_t w = 1; w < kWaves; ++w) {
@@ -601,21 +590,12 @@ __device__ __forceinline__ bool BuildTableInLds(uint32_t f, uint32_t *tab, uint3
       const uint32_t x = sym_of_rank[(upto - 1) & 255];
       const uint32_t w = fc[x];
       const uint32_t slot = wave * kSlotsPerWave + 64 * j + lane;
-      if constexpr (kWide)
-        mine2[64 * j] = make_uint2((w & 0xFFFu) | (x << 24), slot - (w >> 12));
-      else
-        mine[64 * j] = (w & 0xFFFu) | ((slot - (w >> 12)) << 12) | (x << 24);
+      mine[64 * j] = (w & 0xFFFu) | ((slot - (w >> 12)) << 12) | (x << 24);
       base += static_cast<uint32_t>(__popcll(bal));
     }
   } else {
 #pragma unroll
-    for (uint32_t j = 0; j < kChunks; ++j) {
-      const uint32_t slot = wave * kSlotsPerWave + 64 * j + lane;
-      if constexpr (kWide)
-        mine2[64 * j] = make_uint2(kTableSize, slot);
-      else
-        mine[64 * j] = kTableSize | (slot << 12);
-    }
+    for (uint32_t j = 0; j < kChunks; ++j) mine[64 * j] = kTableSize | ((wave * kSlotsPerWave + 64 * j + lane) << 12);
   }
   LdsBarrier();
   return valid;
@@ -627,15 +607,12 @@ __device__ __forceinline__ bool BuildTableInLds(uint32_t f, uint32_t *tab, uint3
 // offsets -> lane states and ring chunks), all from HBM in a full pipeline; the
 // frequencies go out with the offsets and the states and chunks before the table is
 // built, so the LDS build overlaps their latency.
-// kWide (with kBuild): 8-byte table entries, two VALU fewer a step; 16 KiB of table
-// leave LDS for 6 four-wave work-groups per CU instead of 7.
-template <uint32_t WAVES, uint32_t GPW, bool kBuild, bool kWide = false>
-// With 4 or 8 waves LDS allows 7 waves per SIMD (6 with kWide); hold the VGPRs to that.
-__global__ __launch_bounds__(WAVES * 64, WAVES >= 4 ? (kWide ? 6 : 7) : 1)
+template <uint32_t WAVES, uint32_t GPW, bool kBuild>
+// With 4 or 8 waves LDS allows 7 waves per SIMD; hold the VGPRs to that (at most 72).
+__global__ __launch_bounds__(WAVES * 64, WAVES >= 4 ? 7 : 1)
 void AnsDecodeKernel(KernelArgs a) {
-  static_assert(!kWide || (kBuild && WAVES * 2 * kRingWords * 2 >= kTableSize * 4), "wide tables: LDS build, marks in the ring");
-  __shared__ __align__(16) uint32_t tab[kWide ? 2 * kTableSize : kTableSize];
-  __shared__ __align__(16) uint16_t ring[WAVES][2][kRingWords];
+  __shared__ __align__(16) uint32_t tab[kTableSize];
+  __shared__ uint16_t ring[WAVES][2][kRingWords];
   __shared__ uint32_t build_fc[kBuild ? 256 : 1];
   __shared__ uint8_t build_sym[kBuild ? 256 : 1];
   __shared__ uint32_t build_wsum[4];
@@ -690,8 +667,7 @@ void AnsDecodeKernel(KernelArgs a) {
     LdsBarrier();  // every wave is done with the previous unit's table and ring
     bool table_ok;
     if constexpr (kBuild) {
-      table_ok = BuildTableInLds<WAVES * 64, kWide>(f, tab, reinterpret_cast<uint32_t *>(&ring[0][0][0]), build_fc,
-                                                     build_sym, build_wsum, build_before);
+      table_ok = BuildTableInLds<WAVES * 64>(f, tab, build_fc, build_sym, build_wsum, build_before);
     } else {
       const uint32_t tix = Uniform(d.table_base + k);
       const GPtr<const uint32_t> src = Gbl<const uint32_t>(a.tables + static_cast<size_t>(tix) * kTableSize);
@@ -701,17 +677,21 @@ void AnsDecodeKernel(KernelArgs a) {
     }
     if (!table_ok && threadIdx.x == 0) atomicOr(a.err, kErrFrequencies);
 
+    // A wave past its stream's last group (the stream's group count is not a multiple
+    // of the unit) has nothing to decode: it would only take issue slots from the waves
+    // that do.  It goes straight to the next unit's barrier.
+    if (__builtin_amdgcn_ballot_w64(active) == 0) continue;
     const GPtr<uint8_t> outp = d.symbols + SymbolOffset(d, k) + static_cast<size_t>(g) * kGroupSyms + j * kSymsPerLane;
     uint16_t *ring_half = ring[wave][half];
     if (k == 3)
-      DecodeHalfGroups<true, GPW, kWide>(tab, ring[wave][0], ring_half, touch_sink, rs, chunks, lane, active, x, pA, pB,
+      DecodeHalfGroups<true, GPW>(tab, ring[wave][0], ring_half, touch_sink, rs, chunks, lane, active, x, pA, pB,
                                   loA, loB, outp, d.group_sums + g,
                                   d.lane_sums ? d.lane_sums + g * kLanesPerGroup : nullptr,
                                   d.lane_sums ? d.lane_sums + LaneSuffixBase(d.nv) + g * kLanesPerGroup * 8
                                               : nullptr,
                                   a.err);
     else
-      DecodeHalfGroups<false, GPW, kWide>(tab, ring[wave][0], ring_half, touch_sink, rs, chunks, lane, active, x, pA, pB,
+      DecodeHalfGroups<false, GPW>(tab, ring[wave][0], ring_half, touch_sink, rs, chunks, lane, active, x, pA, pB,
                                    loA, loB, outp, nullptr, nullptr, nullptr, a.err);
   }
 }
@@ -1487,9 +1467,9 @@ __global__ __launch_bounds__(256) void AssembleRgbKernel(KernelArgs a) {
   }
 }
 
-template <uint32_t WAVES, uint32_t GPW, bool kBuild = false, bool kWide = false>
+template <uint32_t WAVES, uint32_t GPW, bool kBuild = false>
 hipError_t LaunchAns(const KernelArgs &args, uint32_t grid, hipStream_t stream) {
-  hipLaunchKernelGGL((AnsDecodeKernel<WAVES, GPW, kBuild, kWide>), dim3(grid), dim3(WAVES * 64), 0, stream, args);
+  hipLaunchKernelGGL((AnsDecodeKernel<WAVES, GPW, kBuild>), dim3(grid), dim3(WAVES * 64), 0, stream, args);
   return hipGetLastError();
 }
 
@@ -1505,11 +1485,6 @@ hipError_t LaunchStage(uint32_t stage, const KernelArgs &args, uint32_t grid,
       return hipGetLastError();
     case kStageAns: {
       const uint32_t w = cfg.ans_waves, g = cfg.ans_groups_per_wave;
-      if (cfg.ans_build_tables && cfg.ans_wide_table) {
-        if (g == 2 && w == 4) return LaunchAns<4, 2, true, true>(args, grid, stream);
-        if (g == 2 && w == 8) return LaunchAns<8, 2, true, true>(args, grid, stream);
-        return hipErrorInvalidValue;
-      }
       if (cfg.ans_build_tables) {
         if (g == 2 && w == 4) return LaunchAns<4, 2, true>(args, grid, stream);
         if (g == 2 && w == 8) return LaunchAns<8, 2, true>(args, grid, stream);

@@ -208,8 +208,6 @@ typedef struct gst_batch_config {
                                        (one scan per 1024 blocks) instead of 4 x 4 */
   uint32_t separate_wavelet;        /* 1: the inverse wavelet as its own launch; 0: it runs
                                        inside the DXT1 assembly (planes stay in LDS) */
-  uint32_t ans_wide_table;          /* 1: tables built in LDS as 8-byte entries (two groups
-                                       a wave, 4 or 8 waves per group) */
 } gst_batch_config;
 
 int gst_batch_create(const gst_batch_config *cfg, gst_batch **out);
@@ -246,7 +244,6 @@ typedef struct gst_tune_result {
                                         stage the chosen shapes fold into another launch */
   float total_ms;
   uint32_t wavelet_fused;            /* 1: wavelet + index scan + DXT1 assembly in one kernel */
-  uint32_t ans_wide_table;           /* 1: tables built in LDS as 8-byte entries */
 } gst_tune_result;
 /* No reference counterpart.  Times every launch shape of every stage on the bound
  * batch (median of `reps` launches each, 0 = 5) and keeps the fastest per stage:

@@ -46,22 +46,19 @@ def test_uneven_batch_matches_oracle(device):
         b.close()
 
 
-@pytest.mark.parametrize("gpw,waves,wwaves,awaves,separate,sep_tables,wide,sep_wavelet,wide_table", [
-    (2, 4, 1, 4, False, False, False, True, False), (1, 4, 2, 1, False, True, True, True, False),
-    (2, 8, 4, 2, False, False, False, True, False), (1, 8, 1, 4, True, False, False, True, False),
-    (2, 1, 1, 0, True, False, False, True, False), (1, 2, 4, 0, True, True, False, True, False),
-    (2, 4, 2, 2, False, False, True, True, False), (2, 8, 1, 4, False, True, True, True, False),
-    (2, 4, 1, 4, False, False, False, False, False), (1, 8, 1, 4, False, True, False, False, False),
-    (2, 4, 1, 4, False, False, False, False, True), (2, 8, 1, 4, False, False, False, False, True)])
-def test_launch_shapes_agree(device, gpw, waves, wwaves, awaves, separate, sep_tables, wide, sep_wavelet,
-                             wide_table):
+@pytest.mark.parametrize("gpw,waves,wwaves,awaves,separate,sep_tables,wide,sep_wavelet", [
+    (2, 4, 1, 4, False, False, False, True), (1, 4, 2, 1, False, True, True, True),
+    (2, 8, 4, 2, False, False, False, True), (1, 8, 1, 4, True, False, False, True),
+    (2, 1, 1, 0, True, False, False, True), (1, 2, 4, 0, True, True, False, True),
+    (2, 4, 2, 2, False, False, True, True), (2, 8, 1, 4, False, True, True, True),
+    (2, 4, 1, 4, False, False, False, False), (1, 8, 1, 4, False, True, False, False)])
+def test_launch_shapes_agree(device, gpw, waves, wwaves, awaves, separate, sep_tables, wide, sep_wavelet):
     files, dxts = encode.make_textures(MIXED[:4])
     blocks, pal = _capacity(files)
     b = gst_amd.Batch(len(files), blocks, pal, groups_per_wave=gpw, waves_per_group=waves,
                       grids={"ans_decode": 64, "inv_wavelet": 128, "decode_indices": 16, "assemble": 32},
                       wavelet_waves=wwaves, assembly_waves=awaves, separate_index_scan=separate,
-                      separate_table_stage=sep_tables, assembly_wide=wide, separate_wavelet=sep_wavelet,
-                      wide_table=wide_table)
+                      separate_table_stage=sep_tables, assembly_wide=wide, separate_wavelet=sep_wavelet)
     try:
         _, _, d_outs = _bind(device, b, files)
         b.decode(device.stream)
