@@ -147,6 +147,7 @@ class Gpu:
 
 
 def check_outputs(gpu, d_out, offsets, dxts, what):
+    """Every image's bytes of `d_out` (poisoned before the decode) against its blocks."""
     gpu.torch.cuda.synchronize()
     host = d_out.cpu().numpy()
     for i, (o, d) in enumerate(zip(offsets, dxts)):
@@ -154,14 +155,25 @@ def check_outputs(gpu, d_out, offsets, dxts, what):
             raise SystemExit("%s: image %d differs from the expected DXT1 blocks" % (what, i))
 
 
-def run_batched(gpu, files, dxts, steps, warmup, tune=True):
+LAUNCH_MODES = ("graph", "stream", "two_plans")
+
+
+def run_batched(gpu, files, dxts, steps, warmup, tune=True, modes=LAUNCH_MODES):
+    """The batch decoded K steps after W warm-up steps in each launch mode of `modes`:
+    graph (one hipGraph replay a step), stream (the stage kernels launched on the
+    stream), two_plans (a second plan bound to the same inputs with its own scratch and
+    outputs, on a second stream; step k runs on plan k % 2, so one plan's rANS stage
+    fills the other's assembly tail, as the server's slots run batches).  The step time
+    is the fastest mode.
+    Outputs are poisoned and checked bit-exact after every mode."""
     import gst_amd
     hdrs = [gst_amd.ParseHeader(f) for f in files]
     d_files, file_ptrs = gpu.upload_many(files)
     out_sizes = [h.dxt_bytes for h in hdrs]
     out_offs = list(np.cumsum([0] + out_sizes[:-1]))
     d_out = gpu.empty(sum(out_sizes))
-    b = gst_amd.Batch(len(hdrs), sum(h.num_blocks for h in hdrs), sum(h.palette_bytes for h in hdrs))
+    nv, pal = sum(h.num_blocks for h in hdrs), sum(h.palette_bytes for h in hdrs)
+    b = gst_amd.Batch(len(hdrs), nv, pal)
     b.bind(hdrs, file_ptrs, [d_out.data_ptr() + int(o) for o in out_offs], gpu.stream)
     tuned = None
     if tune:
@@ -169,59 +181,58 @@ def run_batched(gpu, files, dxts, steps, warmup, tune=True):
         tuned = b.autotune(gpu.stream, reps=5)
         tuned["seconds"] = round(time.perf_counter() - t0, 2)
         log("autotune: %s" % tuned)
-        d_out.zero_()
+    d_out.fill_(0xA5)
     b.decode(gpu.stream)
     check_outputs(gpu, d_out, out_offs, dxts, "batched eager")
-    b.capture_graph(gpu.stream)
-    d_out.zero_()
-    b.launch_graph(gpu.stream)
-    check_outputs(gpu, d_out, out_offs, dxts, "batched graph")
     if b.error_bits(gpu.stream):
         raise SystemExit("batched decode raised device error bits")
-    # Both launch paths of the batch API, K steps each after W warm-up steps each:
-    # gst_batch_launch_graph (one hipGraph replay a step) and gst_batch_decode (the
-    # stage kernels launched on the stream).  The step time is the faster path; both
-    # are reported.
-    for _ in range(warmup):
+    launch = {}
+    if "graph" in modes:
+        b.capture_graph(gpu.stream)
+        for _ in range(warmup):
+            b.launch_graph(gpu.stream)
+        launch["graph"] = gpu.timed(lambda: b.launch_graph(gpu.stream), steps)
+        d_out.fill_(0xA5)
         b.launch_graph(gpu.stream)
+        check_outputs(gpu, d_out, out_offs, dxts, "batched graph")
+    if "stream" in modes:
+        for _ in range(warmup):
+            b.decode(gpu.stream)
+        launch["stream"] = gpu.timed(lambda: b.decode(gpu.stream), steps)
+        d_out.fill_(0xA5)
         b.decode(gpu.stream)
-    launch = {"graph": gpu.timed(lambda: b.launch_graph(gpu.stream), steps),
-              "stream": gpu.timed(lambda: b.decode(gpu.stream), steps)}
-    d_out.zero_()
-    b.decode(gpu.stream)
-    check_outputs(gpu, d_out, out_offs, dxts, "batched stream launches")
-    # Two batches in flight: a second plan bound to the same inputs, with its own scratch
-    # and outputs, on a second stream; step k runs on plan k % 2, so one plan's rANS
-    # stage fills the other's assembly tail (the server's slots run batches this way).
-    s0 = gpu.torch.cuda.current_stream()
-    s1 = gpu.torch.cuda.Stream(device=gpu.dev)
-    d_out2 = gpu.empty(sum(out_sizes))
-    b2 = gst_amd.Batch(len(hdrs), sum(h.num_blocks for h in hdrs), sum(h.palette_bytes for h in hdrs))
-    b2.bind(hdrs, file_ptrs, [d_out2.data_ptr() + int(o) for o in out_offs], s1.cuda_stream)
-    if tune:
-        b2.autotune(s1.cuda_stream, reps=5)
-    plans = ((b, s0), (b2, s1))
+        check_outputs(gpu, d_out, out_offs, dxts, "batched stream launches")
+    if "two_plans" in modes:
+        s0 = gpu.torch.cuda.current_stream()
+        s1 = gpu.torch.cuda.Stream(device=gpu.dev)
+        d_out2 = gpu.empty(sum(out_sizes))
+        b2 = gst_amd.Batch(len(hdrs), nv, pal)
+        b2.bind(hdrs, file_ptrs, [d_out2.data_ptr() + int(o) for o in out_offs], s1.cuda_stream)
+        if tune:
+            b2.autotune(s1.cuda_stream, reps=5)
+        plans = ((b, s0), (b2, s1))
 
-    def two_plans(k_steps):
-        for k in range(k_steps):
-            plan, st = plans[k % 2]
-            plan.decode(st.cuda_stream)
+        def two_plans(k_steps):
+            for k in range(k_steps):
+                plan, st = plans[k % 2]
+                plan.decode(st.cuda_stream)
 
-    two_plans(2 * warmup)
-    launch["two_plans"] = gpu.timed_two(s0, s1, lambda: two_plans(steps), steps)
-    d_out.zero_()
-    d_out2.zero_()
-    two_plans(2)
-    gpu.torch.cuda.synchronize()
-    check_outputs(gpu, d_out, out_offs, dxts, "two plans, first")
-    check_outputs(gpu, d_out2, out_offs, dxts, "two plans, second")
-    b2.close()
-    del d_out2
+        two_plans(2 * warmup)
+        launch["two_plans"] = gpu.timed_two(s0, s1, lambda: two_plans(steps), steps)
+        d_out.fill_(0xA5)
+        d_out2.fill_(0xA5)
+        two_plans(2)
+        gpu.torch.cuda.synchronize()
+        check_outputs(gpu, d_out, out_offs, dxts, "two plans, first")
+        check_outputs(gpu, d_out2, out_offs, dxts, "two plans, second")
+        b2.close()
+        del d_out2
     ms = min(launch.values())
     # Per-stage kernel time in the pipeline's order and cache state: the five launches
     # with a hipEvent recorded on the decode stream before and after each, mean of 20.
     reps = 20
     stage = {k: None for k in gst_amd.STAGES}
+    d_out.fill_(0xA5)
     for _ in range(reps):
         for k, v in b.decode_timed(gpu.stream).items():
             if v is not None:
@@ -520,6 +531,8 @@ def main():
     ap.add_argument("--served-seconds", type=float, default=3.0)
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes (no autotune)")
     ap.add_argument("--no-ans-api", action="store_true", help="skip the stand-alone rANS API leg")
+    ap.add_argument("--launch-modes", default=",".join(LAUNCH_MODES),
+                    help="comma list of the batch launch modes to time (default: all): %s" % ",".join(LAUNCH_MODES))
     ap.add_argument("--stage-kernels", action="store_true",
                     help="only time the five per-stage kernels (gst_stage_*) on the 512x512 and 4096x4096 "
                          "single-image configurations and print their GB/s")
@@ -557,8 +570,11 @@ def main():
     enc_seconds = time.perf_counter() - t0
     log("corpus: %d textures encoded in %.1fs" % (len(files), enc_seconds))
 
+    modes = tuple(m for m in args.launch_modes.split(",") if m)
+    if not modes or any(m not in LAUNCH_MODES for m in modes):
+        raise SystemExit("--launch-modes takes a comma list of %s" % ",".join(LAUNCH_MODES))
     hdrs, ms, stage, dev_bytes, tuned, launch = run_batched(gpu, files, dxts, args.steps, args.warmup,
-                                                            tune=not args.no_tune)
+                                                            tune=not args.no_tune, modes=modes)
     texels = sum(h.width * h.height for h in hdrs)
     value = texels / (ms * 1e-3) / 1e6
     log("batched: %.3f ms/step  %.0f Mtexel/s  stages %s" % (ms, value, stage))
@@ -591,7 +607,8 @@ def main():
             "workload": "C3 batched: %d seeded textures {512x512,1024x1024,512x1024,2048x2048}, "
                         "one pass of the persistent stage kernels per step, %s"
                         % (len(hdrs), {"graph": "hipGraph replays", "stream": "stream launches",
-                                       "two_plans": "two plans in flight on two streams"}[min(launch, key=launch.get)]),
+                                       "two_plans": "two plans in flight on two streams"}
+                           [min(launch, key=launch.get)]),
             "images": len(hdrs),
             "texels": texels,
             "dxt1_blocks": texels // 16,
