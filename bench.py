@@ -75,7 +75,8 @@ def stage_bytes(hdrs, mode="tile_fused", tables_in_lds=True):
 
 def measured_traffic(stage, n_images, requested):
     """HBM bytes per launch of `stage` (read + write) from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary of this workload, scaled by the copy calibration;
+    summary of this workload (L2-to-memory requests by size: 128 x RDREQ_128B + 64 x
+    RDREQ_64B + 32 x RDREQ_32B read, 64 x WRREQ_64B + 32 x the other write requests);
     None for another corpus size or a stage the summary lacks."""
     path = os.path.join(ROOT, "profiles", "c3_batch_traffic.json")
     if requested != 128 or n_images != 128 or not os.path.exists(path):
