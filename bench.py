@@ -538,8 +538,8 @@ def main():
                     help="only time the five per-stage kernels (gst_stage_*) on the 512x512 and 4096x4096 "
                          "single-image configurations and print their GB/s")
     ap.add_argument("--copy-calibration", action="store_true",
-                    help="only run a device-to-device copy of 512 MiB (5x), the reference kernel for "
-                         "scaling FETCH_SIZE / WRITE_SIZE counters to bytes")
+                    help="only run a device-to-device copy of 512 MiB (5x): a kernel of known traffic "
+                         "against which HBM byte counters are checked")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (--gpus %d requested)" % args.gpus,
