@@ -69,7 +69,7 @@ def build_encoder():
 def build_cpp_tests():
     """tests/cpp/codec_test and tests/cpp/ans_ocl_test: the reference's end-to-end decoder
     test and its ANS decoder tests against the C++ API; tests/cpp/encoder_test: the
-    encoder's C++ API."""
+    encoder's C++ API; tests/cpp/ans_units_test: the rANS work-unit enumeration."""
     outs = []
     for name in ("codec_test", "ans_ocl_test"):
         src = os.path.join(ROOT, "tests", "cpp", name + ".cpp")
@@ -83,6 +83,11 @@ def build_cpp_tests():
     outs.append(_atomic_build(os.path.join(ROOT, "tests", "cpp", "encoder_test"), lambda o: [
         cxx, "-std=c++17", "-O2", "-Wall", "-Wextra", "-I" + INCLUDE, src, "-L" + HERE, "-lgst_encode",
         "-Wl,-rpath," + HERE, "-o", o]))
+    # The rANS work-unit enumeration the host and the kernels share: host code only.
+    src = os.path.join(ROOT, "tests", "cpp", "ans_units_test.cpp")
+    outs.append(_atomic_build(os.path.join(ROOT, "tests", "cpp", "ans_units_test"), lambda o: [
+        cxx, "-std=c++17", "-O2", "-Wall", "-Wextra", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+        "-I" + CSRC, "-I" + INCLUDE, src, "-o", o]))
     return outs
 
 

@@ -148,11 +148,15 @@ StageConfig DefaultStageConfig(bool rgb) {
   s.fused_waves = rgb ? 0 : 4;
   s.tile_fused = rgb ? 0 : 1;
   s.ans_build_tables = 1;
+  s.ans_pair_short = 1;
   s.rgb = rgb ? 1 : 0;
   return s;
 }
 
-uint32_t GroupsPerUnit(const StageConfig &s) { return s.ans_waves * s.ans_groups_per_wave; }
+// Groups per rANS unit, with gst::kAnsPairShort when short remainders pair up.
+uint32_t AnsUnitCode(const StageConfig &s) {
+  return s.ans_waves * s.ans_groups_per_wave | (s.ans_pair_short ? gst::kAnsPairShort : 0u);
+}
 
 uint32_t *ErrWord(hipStream_t s) {
   Globals &g = G();
@@ -223,20 +227,20 @@ class PinnedRing {
 
 // The used part of a plan of n <= cap images (layout in gst_internal.h): its size is
 // PlanUsedBytes(cap, n).
-void BuildPlan(const std::vector<ImageDesc> &descs, uint32_t cap, uint32_t groups_per_unit,
+void BuildPlan(const std::vector<ImageDesc> &descs, uint32_t cap, uint32_t ans_unit_code,
                std::vector<uint8_t> *blob, uint32_t units_out[gst::kNumStages]) {
   const uint32_t n = static_cast<uint32_t>(descs.size());
   blob->assign(gst::PlanUsedBytes(cap, n), 0);
   PlanHeader *ph = reinterpret_cast<PlanHeader *>(blob->data());
   ImageDesc *im = reinterpret_cast<ImageDesc *>(blob->data() + gst::PlanImagesOffset(cap));
   ph->n_images = n;
-  ph->groups_per_unit = groups_per_unit;
+  ph->ans_unit_code = ans_unit_code;
   for (uint32_t st = 0; st < gst::kNumStages; ++st) {
     uint32_t *pre = reinterpret_cast<uint32_t *>(blob->data() + gst::PlanPrefixOffset(cap, st));
     uint32_t acc = 0;
     for (uint32_t i = 0; i < n; ++i) {
       pre[i] = acc;
-      acc += gst::StageUnitsOf(descs[i], st, groups_per_unit);
+      acc += gst::StageUnitsOf(descs[i], st, ans_unit_code);
     }
     pre[n] = acc;
     for (uint32_t i = n + 1; i <= cap; ++i) pre[i] = gst::kPrefixPad;
@@ -359,12 +363,12 @@ int DecodeStaging(const gst_header *hdrs, uint32_t n, hipStream_t s, const void 
     a.tables = tables;
     a.table_status = status;
     for (uint32_t st = 0; st < gst::kNumStages; ++st)
-      grids[st] = gst::SinglePassGrid(st, gst::StageUnitsOf(descs[0], st, GroupsPerUnit(cfg)), cfg);
+      grids[st] = gst::SinglePassGrid(st, gst::StageUnitsOf(descs[0], st, AnsUnitCode(cfg)), cfg);
     rc = LaunchAll(a, grids, cfg, s);
   } else {
     std::vector<uint8_t> blob;
     uint32_t units[gst::kNumStages];
-    BuildPlan(descs, n, GroupsPerUnit(cfg), &blob, units);
+    BuildPlan(descs, n, AnsUnitCode(cfg), &blob, units);
     uint8_t *d_blob = sc.ptr + L.plan;
     rc = PinnedRing::Get().Upload(d_blob, blob.data(), blob.size(), s);
     if (rc == GST_OK) {
@@ -574,7 +578,7 @@ int gst_load_compressed_dxt_noalloc(const gst_header *hdr, gst_stream stream, co
   a.err = d_err_bits;
   uint32_t grids[gst::kNumStages];
   for (uint32_t st = 0; st < gst::kNumStages; ++st)
-    grids[st] = gst::SinglePassGrid(st, gst::StageUnitsOf(a.single, st, GroupsPerUnit(cfg)), cfg);
+    grids[st] = gst::SinglePassGrid(st, gst::StageUnitsOf(a.single, st, AnsUnitCode(cfg)), cfg);
   return LaunchAll(a, grids, cfg, static_cast<hipStream_t>(stream));
 }
 
@@ -677,7 +681,7 @@ KernelArgs BatchArgs(gst_batch *b) {
 // copies it to the device on `s` through the pinned staging buffer.
 int UploadPlan(gst_batch *b, hipStream_t s, uint32_t units[gst::kNumStages]) {
   std::vector<uint8_t> blob;
-  BuildPlan(b->bound, b->cfg.capacity_images, GroupsPerUnit(b->stage), &blob, units);
+  BuildPlan(b->bound, b->cfg.capacity_images, AnsUnitCode(b->stage), &blob, units);
   if (b->blob_pending) GST_HIP(hipEventSynchronize(b->blob_ev));
   memcpy(b->h_blob, blob.data(), blob.size());
   GST_HIP(hipMemcpyAsync(b->dmem + b->plan_off, b->h_blob, blob.size(), hipMemcpyHostToDevice, s));
@@ -756,6 +760,10 @@ int gst_batch_create(const gst_batch_config *cfg, gst_batch **out) {
   b->stage.ans_waves = cfg->ans_waves_per_group ? cfg->ans_waves_per_group : 4;
   b->stage.wavelet_waves = cfg->wavelet_waves_per_group ? cfg->wavelet_waves_per_group : 1;
   b->stage.ans_build_tables = (cfg->separate_table_stage || b->stage.ans_waves < 4) ? 0 : 1;
+  b->stage.ans_pair_short = (b->stage.ans_build_tables && b->stage.ans_waves == 4 &&
+                             b->stage.ans_groups_per_wave == 2 && !cfg->ans_single_stream_units)
+                                ? 1
+                                : 0;
   b->stage.fused_waves = (cfg->rgb_output || cfg->separate_index_scan)
                              ? 0
                              : (cfg->assembly_waves_per_group ? cfg->assembly_waves_per_group : 4);
@@ -845,23 +853,28 @@ int gst_batch_autotune(gst_batch *b, gst_stream stream, uint32_t reps, gst_tune_
   // Tables as their own launch (one work-group per table); the rANS shapes that load
   // them are charged this time, the ones that build their table in LDS are not.
   best.ans_build_tables = 0;
+  best.ans_pair_short = 0;
   grids[gst::kStageTables] = 4 * b->n_bound;
   float t_tables = 0.f;
   if ((rc = TimeStage(b, gst::kStageTables, grids[gst::kStageTables], best, s, reps, &t_tables)) != GST_OK)
     return rc;
 
   // rANS: streams per wavefront (32 = one group per wave, 64 = two) x waves per
-  // work-group x table source x persistent grid; each shape needs its own plan
-  // (units of W*GPW groups).
+  // work-group x table source (mode 0: loaded, 1: built in LDS, 2: built in LDS with
+  // short remainders paired) x persistent grid; each shape needs its own plan (units
+  // of W*GPW groups).
   static const uint32_t kAnsShapes[][2] = {{1, 2}, {1, 4}, {1, 8}, {2, 1}, {2, 2}, {2, 4}, {2, 8}};
   float best_ms = 1e30f, best_ans = 0.f;
   for (const auto &shape : kAnsShapes) {
-    for (uint32_t build = 0; build < 2; ++build) {
+    for (uint32_t mode = 0; mode < 3; ++mode) {
+      const uint32_t build = mode >= 1 ? 1u : 0u, pair = mode == 2 ? 1u : 0u;
       if (build && shape[1] < 4) continue;  // the LDS build needs 256 threads
+      if (pair && !(shape[0] == 2 && shape[1] == 4)) continue;
       StageConfig c = best;
       c.ans_groups_per_wave = shape[0];
       c.ans_waves = shape[1];
       c.ans_build_tables = build;
+      c.ans_pair_short = pair;
       b->stage = c;
       if ((rc = UploadPlan(b, s, units)) != GST_OK) return rc;
       const uint32_t cand[] = {cus, 2 * cus, 4 * cus, 8 * cus, units[gst::kStageAns]};
@@ -875,6 +888,7 @@ int gst_batch_autotune(gst_batch *b, gst_stream stream, uint32_t reps, gst_tune_
           best.ans_groups_per_wave = c.ans_groups_per_wave;
           best.ans_waves = c.ans_waves;
           best.ans_build_tables = build;
+          best.ans_pair_short = pair;
           grids[gst::kStageAns] = g;
         }
       }
@@ -1006,6 +1020,7 @@ int gst_batch_autotune(gst_batch *b, gst_stream stream, uint32_t reps, gst_tune_
   res.wavelet_waves_per_group = best.wavelet_waves;
   res.assembly_waves_per_group = best.fused_waves;
   res.ans_builds_tables = best.ans_build_tables;
+  res.ans_pairs_short = best.ans_pair_short;
   res.assembly_wide = best.fused_wide;
   res.wavelet_fused = best.tile_fused;
   res.grid_ans = grids[gst::kStageAns];
@@ -1133,7 +1148,8 @@ int StageLaunch(uint32_t stage, const gst_header *hdr, const uint8_t *d_freqs, c
   cfg.fused_waves = 0;  // each stage is its own launch here
   cfg.tile_fused = 0;
   cfg.ans_build_tables = 0;
-  const uint32_t grid = gst::SinglePassGrid(stage, gst::StageUnitsOf(d, stage, GroupsPerUnit(cfg)), cfg);
+  cfg.ans_pair_short = 0;
+  const uint32_t grid = gst::SinglePassGrid(stage, gst::StageUnitsOf(d, stage, AnsUnitCode(cfg)), cfg);
   GST_HIP(gst::LaunchStage(stage, a, grid, cfg, s));
   return GST_OK;
 }

@@ -68,17 +68,83 @@ __host__ __device__ inline uint32_t SymbolOffset(const Desc &d, uint32_t k) {
   return k == 0 ? 0u : k == 1 ? 2 * d.nv : k == 2 ? 6 * d.nv : 6 * d.nv + d.palette_bytes;
 }
 
-__host__ __device__ inline uint32_t AnsUnitsOf(const ImageDesc &d, uint32_t groups_per_unit) {
-  uint32_t u = 0;
-  for (uint32_t k = 0; k < 4; ++k) u += (d.groups[k] + groups_per_unit - 1) / groups_per_unit;
-  return u;
+// rANS work units.  A unit is up to `per` consecutive groups of one stream (per = waves
+// x groups per wave; the low 8 bits of the unit code), decoded by one work-group.  With
+// kAnsPairShort in the code, a stream's remainder of at most per / 2 groups gets no
+// work-group of its own: an image's short remainders go two to a work-group, one per
+// half of its waves, each half with its own table.  Fewer, fuller work-groups then fill
+// the chip in one pass.  An image's units are every stream's full units and long
+// remainder, in stream order, then the pairs of short remainders.
+constexpr uint32_t kAnsPairShort = 0x100;
+
+__host__ __device__ inline bool AnsShortRemainder(uint32_t groups, uint32_t code) {
+  const uint32_t per = code & 0xFFu, rem = groups % per;
+  return (code & kAnsPairShort) != 0 && rem != 0 && rem <= per / 2;
 }
 
+// Units of stream k that are not short remainders.
+__host__ __device__ inline uint32_t AnsStreamUnits(uint32_t groups, uint32_t code) {
+  const uint32_t per = code & 0xFFu;
+  return groups / per + (groups % per != 0 && !AnsShortRemainder(groups, code) ? 1u : 0u);
+}
+
+template <typename Desc>
+__host__ __device__ inline uint32_t AnsUnitsOf(const Desc &d, uint32_t code) {
+  uint32_t u = 0, shorts = 0;
+  for (uint32_t k = 0; k < 4; ++k) {
+    u += AnsStreamUnits(d.groups[k], code);
+    shorts += AnsShortRemainder(d.groups[k], code) ? 1u : 0u;
+  }
+  return u + (shorts + 1) / 2;
+}
+
+// Unit `local` of an image: one segment (k[0], first[0], count[0]) or, for a pair of
+// short remainders, two (count[1] = 0 when the image's last pair has one).
+struct AnsUnit {
+  uint32_t k[2], first[2], count[2];
+  bool pair;
+};
+
+template <typename Desc>
+__host__ __device__ inline AnsUnit AnsUnitAt(const Desc &d, uint32_t code, uint32_t local) {
+  const uint32_t per = code & 0xFFu;
+  AnsUnit r = {{0u, 0u}, {0u, 0u}, {0u, 0u}, false};
+  for (uint32_t k = 0; k < 4; ++k) {
+    const uint32_t g = d.groups[k], nk = AnsStreamUnits(g, code);
+    if (local < nk) {
+      r.k[0] = k;
+      r.first[0] = local * per;
+      r.count[0] = g - local * per < per ? g - local * per : per;
+      return r;
+    }
+    local -= nk;
+  }
+  r.pair = true;
+  uint32_t idx = 0;
+  for (uint32_t k = 0; k < 4; ++k) {
+    const uint32_t g = d.groups[k];
+    if (!AnsShortRemainder(g, code)) continue;
+    // Constant subscripts only: a computed one would put the unit in scratch memory.
+    if (idx == 2 * local) {
+      r.k[0] = k;
+      r.first[0] = g - g % per;
+      r.count[0] = g % per;
+    } else if (idx == 2 * local + 1) {
+      r.k[1] = k;
+      r.first[1] = g - g % per;
+      r.count[1] = g % per;
+    }
+    ++idx;
+  }
+  return r;
+}
+
+// `ans_unit_code`: groups per rANS unit, with kAnsPairShort when short remainders pair up.
 __host__ __device__ inline uint32_t StageUnitsOf(const ImageDesc &d, uint32_t stage,
-                                                 uint32_t groups_per_unit) {
+                                                 uint32_t ans_unit_code) {
   switch (stage) {
     case kStageTables: return 4;
-    case kStageAns: return AnsUnitsOf(d, groups_per_unit);
+    case kStageAns: return AnsUnitsOf(d, ans_unit_code);
     case kStageIndices: return d.nv / kGroupSyms;
     case kStageWavelet: return 6 * (d.nv / kTileSyms) / kWaveletTilesPerUnit;
     default: return d.nv / kAssembleBlocksPerUnit;
@@ -95,7 +161,7 @@ __host__ __device__ inline uint32_t StageUnitsOf(const ImageDesc &d, uint32_t st
 struct PlanHeader {
   uint32_t n_images;
   uint32_t units[kNumStages];
-  uint32_t groups_per_unit;
+  uint32_t ans_unit_code;  // groups per rANS unit | kAnsPairShort
 };
 constexpr size_t kPlanHeaderBytes = 64;
 static_assert(sizeof(PlanHeader) <= kPlanHeaderBytes, "plan header");
@@ -126,8 +192,10 @@ struct KernelArgs {
 struct StageConfig {
   uint32_t ans_waves;            // waves per rANS work-group (1, 2, 4, 8)
   uint32_t ans_groups_per_wave;  // 1 or 2
-  uint32_t ans_build_tables;     // 1: the rANS kernel builds its table in LDS (4 or 8 waves);
-                                 // the table stage launches nothing
+  uint32_t ans_build_tables;     // 1: the rANS kernel builds its table in LDS (4 or 8 waves,
+                                 // or 2 waves of two groups); the table stage launches nothing
+  uint32_t ans_pair_short;       // 1: short stream remainders pair up in one work-group
+                                 // (4 waves x 2 groups, tables built in LDS)
   uint32_t wavelet_waves;        // waves per wavelet work-group (1, 2, 4); a wave takes 2 units
   uint32_t fused_waves;          // 0: separate index scan and assembly; else waves per
                                  // work-group of the fused scan + DXT1 assembly (1, 2, 4)

@@ -603,54 +603,69 @@ __device__ __forceinline__ bool BuildTableInLds(uint32_t f, uint32_t *tab, uint3
 
 // kBuild: build the stream's table from its frequencies in LDS instead of loading
 // the one BuildTableKernel wrote (the decode pipeline then skips that launch).
+// kPair: units are numbered with short remainders paired (AnsUnitAt); a pair's two
+// streams each get half of the waves and a table of their own, built one after the
+// other.  The second table takes 8 KiB of LDS, which caps the kernel at 6 work-groups
+// per CU: that is the point, a SIMD then never holds more than 6 decoding waves.
 // A work-unit starts with a chain of dependent loads (plan -> descriptor -> group
 // offsets -> lane states and ring chunks), all from HBM in a full pipeline; the
 // frequencies go out with the offsets and the states and chunks before the table is
 // built, so the LDS build overlaps their latency.
-template <uint32_t WAVES, uint32_t GPW, bool kBuild>
-// With 4 or 8 waves LDS allows 7 waves per SIMD; hold the VGPRs to that (at most 72).
-__global__ __launch_bounds__(WAVES * 64, WAVES >= 4 ? 7 : 1)
+template <uint32_t WAVES, uint32_t GPW, bool kBuild, bool kPair = false>
+// With 4 or 8 waves LDS allows 7 waves per SIMD (6 with the second table); hold the
+// VGPRs to that (at most 72, or 80).
+__global__ __launch_bounds__(WAVES * 64, kPair ? 6 : WAVES >= 4 ? 7 : 1)
 void AnsDecodeKernel(KernelArgs a) {
-  __shared__ __align__(16) uint32_t tab[kTableSize];
-  __shared__ uint16_t ring[WAVES][2][kRingWords];
-  __shared__ uint32_t build_fc[kBuild ? 256 : 1];
-  __shared__ uint8_t build_sym[kBuild ? 256 : 1];
+  static_assert(!kPair || (kBuild && WAVES % 2 == 0), "paired units build their tables in LDS");
+  __shared__ __align__(16) uint32_t tab[kPair ? 2 : 1][kTableSize];
+  __shared__ __align__(16) uint16_t ring[WAVES][2][kRingWords];
+  // The table build's scratch lives in the rings: every wave has left the previous
+  // unit's ring (barrier) before the build, and the rings are filled after it.
+  static_assert(WAVES * 2 * kRingWords * 2 >= 256 * 4 + 256, "build scratch fits the rings");
+  uint32_t *const build_fc = reinterpret_cast<uint32_t *>(&ring[0][0][0]);
+  uint8_t *const build_sym = reinterpret_cast<uint8_t *>(build_fc + 256);
   __shared__ uint32_t build_wsum[4];
   __shared__ uint32_t build_before[WAVES];
   __shared__ uint32_t touch_sink[64];
   constexpr uint32_t kGroupsPerUnit = WAVES * GPW;
+  constexpr uint32_t kUnitCode = kGroupsPerUnit | (kPair ? kAnsPairShort : 0u);
   const uint32_t lane = threadIdx.x & 63, wave = Uniform(threadIdx.x >> 6);
   const uint32_t half = GPW == 2 ? (lane >> 5) : 0;
   const uint32_t j = lane & 31;
   for (uint32_t u = blockIdx.x;; u += gridDim.x) {
-    const Located loc = Locate(a, kStageAns, u, kGroupsPerUnit);
+    const Located loc = Locate(a, kStageAns, u, kUnitCode);
     if (!loc.valid) break;
     const DevDesc d = LoadDesc(a, loc.img);
-    uint32_t k = 0, local = loc.local;
-    for (; k < 3; ++k) {
-      const uint32_t uk = (Pick4(d.groups, k) + kGroupsPerUnit - 1) / kGroupsPerUnit;
-      if (local < uk) break;
-      local -= uk;
-    }
-    k = Uniform(k);
-    local = Uniform(local);
+    const AnsUnit unit = AnsUnitAt(d, kUnitCode, Uniform(loc.local));
+    // This wave's segment: the unit's only one, or its half of a pair.
+    const uint32_t seg = kPair && unit.pair && wave >= WAVES / 2 ? 1u : 0u;
+    const uint32_t k = Uniform(seg ? unit.k[1] : unit.k[0]);
+    const uint32_t first = Uniform(seg ? unit.first[1] : unit.first[0]);
+    const uint32_t count = Uniform(seg ? unit.count[1] : unit.count[0]);
+    const uint32_t gi = (kPair && unit.pair ? wave % (WAVES / 2) : wave) * GPW + half;
     const uint32_t G = Uniform(Pick4(d.groups, k));
-    const uint32_t g = local * kGroupsPerUnit + wave * GPW + half;
+    const uint32_t g = first + gi;
     const uint32_t slen = Uniform(Pick4(d.stream_len, k));
     const __amdgpu_buffer_rsrc_t rs = MakeRsrc(Uniform(Pick4(d.streams, k)), slen);
     // Straight-line loads (out-of-range offsets read zeros): a load under a branch
     // makes the wait for it land at the join, after later loads have gone out.
     const uint32_t end = __builtin_amdgcn_raw_buffer_load_b32(rs, static_cast<int>(4 * g), 0, 0);
     const uint32_t lo_word = __builtin_amdgcn_raw_buffer_load_b32(rs, static_cast<int>(4 * (g - 1)), 0, 0);
-    uint32_t f = 0;
+    // Frequencies of the unit's table(s): thread t brings symbol t's.
+    uint32_t f = 0, f2 = 0;
+    const bool second = kPair && unit.pair && unit.count[1] != 0;
     if constexpr (kBuild) {
-      const GPtr<const uint16_t> fr = Gbl<const uint16_t>(Uniform(d.freqs + 512 * k));
+      const GPtr<const uint16_t> fr = Gbl<const uint16_t>(Uniform(d.freqs + 512 * unit.k[0]));
       f = fr[threadIdx.x & 255];
+      if (kPair && second) {
+        const GPtr<const uint16_t> fr2 = Gbl<const uint16_t>(Uniform(d.freqs + 512 * unit.k[1]));
+        f2 = fr2[threadIdx.x & 255];
+      }
     }
     const uint32_t lo = g == 0 ? 4 * G : lo_word;
     const bool ok = ((end | lo) & 3) == 0 && end <= slen && lo >= 4 * G && lo <= end &&
                     end - lo >= 4 * kLanesPerGroup;
-    const bool in_stream = (GPW == 2 || lane < 32) && g < G;
+    const bool in_stream = (GPW == 2 || lane < 32) && gi < count;
     if (in_stream && !ok && j == 0) atomicOr(a.err, kErrStreamOffsets);
     const bool active = in_stream && ok;
     const uint32_t x_word =
@@ -664,14 +679,17 @@ void AnsDecodeKernel(KernelArgs a) {
     const uint32_t loB = GPW == 2 ? __builtin_amdgcn_readlane(plo, 32) : 0u;
     const AnsRing chunks = RequestRing<GPW>(rs, pA, pB, loA, loB, lane);
 
-    LdsBarrier();  // every wave is done with the previous unit's table and ring
+    LdsBarrier();  // every wave is done with the previous unit's tables and rings
     bool table_ok;
     if constexpr (kBuild) {
-      table_ok = BuildTableInLds<WAVES * 64>(f, tab, build_fc, build_sym, build_wsum, build_before);
+      table_ok = BuildTableInLds<WAVES * 64>(f, tab[0], build_fc, build_sym, build_wsum, build_before);
+      if (kPair && second)
+        table_ok = BuildTableInLds<WAVES * 64>(f2, tab[kPair ? 1 : 0], build_fc, build_sym, build_wsum,
+                                               build_before) && table_ok;
     } else {
       const uint32_t tix = Uniform(d.table_base + k);
       const GPtr<const uint32_t> src = Gbl<const uint32_t>(a.tables + static_cast<size_t>(tix) * kTableSize);
-      for (uint32_t i = threadIdx.x; i < kTableSize; i += WAVES * 64) tab[i] = src[i];
+      for (uint32_t i = threadIdx.x; i < kTableSize; i += WAVES * 64) tab[0][i] = src[i];
       table_ok = Gbl<const uint32_t>(a.table_status)[tix] != 0;
       LdsBarrier();
     }
@@ -681,17 +699,18 @@ void AnsDecodeKernel(KernelArgs a) {
     // of the unit) has nothing to decode: it would only take issue slots from the waves
     // that do.  It goes straight to the next unit's barrier.
     if (__builtin_amdgcn_ballot_w64(active) == 0) continue;
+    const uint32_t *const tw = tab[kPair ? seg : 0];
     const GPtr<uint8_t> outp = d.symbols + SymbolOffset(d, k) + static_cast<size_t>(g) * kGroupSyms + j * kSymsPerLane;
     uint16_t *ring_half = ring[wave][half];
     if (k == 3)
-      DecodeHalfGroups<true, GPW>(tab, ring[wave][0], ring_half, touch_sink, rs, chunks, lane, active, x, pA, pB,
+      DecodeHalfGroups<true, GPW>(tw, ring[wave][0], ring_half, touch_sink, rs, chunks, lane, active, x, pA, pB,
                                   loA, loB, outp, d.group_sums + g,
                                   d.lane_sums ? d.lane_sums + g * kLanesPerGroup : nullptr,
                                   d.lane_sums ? d.lane_sums + LaneSuffixBase(d.nv) + g * kLanesPerGroup * 8
                                               : nullptr,
                                   a.err);
     else
-      DecodeHalfGroups<false, GPW>(tab, ring[wave][0], ring_half, touch_sink, rs, chunks, lane, active, x, pA, pB,
+      DecodeHalfGroups<false, GPW>(tw, ring[wave][0], ring_half, touch_sink, rs, chunks, lane, active, x, pA, pB,
                                    loA, loB, outp, nullptr, nullptr, nullptr, a.err);
   }
 }
@@ -1467,9 +1486,9 @@ __global__ __launch_bounds__(256) void AssembleRgbKernel(KernelArgs a) {
   }
 }
 
-template <uint32_t WAVES, uint32_t GPW, bool kBuild = false>
+template <uint32_t WAVES, uint32_t GPW, bool kBuild = false, bool kPair = false>
 hipError_t LaunchAns(const KernelArgs &args, uint32_t grid, hipStream_t stream) {
-  hipLaunchKernelGGL((AnsDecodeKernel<WAVES, GPW, kBuild>), dim3(grid), dim3(WAVES * 64), 0, stream, args);
+  hipLaunchKernelGGL((AnsDecodeKernel<WAVES, GPW, kBuild, kPair>), dim3(grid), dim3(WAVES * 64), 0, stream, args);
   return hipGetLastError();
 }
 
@@ -1485,6 +1504,10 @@ hipError_t LaunchStage(uint32_t stage, const KernelArgs &args, uint32_t grid,
       return hipGetLastError();
     case kStageAns: {
       const uint32_t w = cfg.ans_waves, g = cfg.ans_groups_per_wave;
+      if (cfg.ans_pair_short) {
+        if (cfg.ans_build_tables && g == 2 && w == 4) return LaunchAns<4, 2, true, true>(args, grid, stream);
+        return hipErrorInvalidValue;
+      }
       if (cfg.ans_build_tables) {
         if (g == 2 && w == 4) return LaunchAns<4, 2, true>(args, grid, stream);
         if (g == 2 && w == 8) return LaunchAns<8, 2, true>(args, grid, stream);

@@ -208,6 +208,11 @@ typedef struct gst_batch_config {
                                        (one scan per 1024 blocks) instead of 4 x 4 */
   uint32_t separate_wavelet;        /* 1: the inverse wavelet as its own launch; 0: it runs
                                        inside the DXT1 assembly (planes stay in LDS) */
+  uint32_t ans_single_stream_units; /* 1: every rANS work-group decodes groups of one stream;
+                                       0: with 4 waves x 2 groups and tables built in LDS, a
+                                       stream's remainder of at most 4 groups shares a
+                                       work-group with another stream's (fewer work-groups, at
+                                       most 6 per CU) */
 } gst_batch_config;
 
 int gst_batch_create(const gst_batch_config *cfg, gst_batch **out);
@@ -244,6 +249,7 @@ typedef struct gst_tune_result {
                                         stage the chosen shapes fold into another launch */
   float total_ms;
   uint32_t wavelet_fused;            /* 1: wavelet + index scan + DXT1 assembly in one kernel */
+  uint32_t ans_pairs_short;          /* 1: short stream remainders share rANS work-groups */
 } gst_tune_result;
 /* No reference counterpart.  Times every launch shape of every stage on the bound
  * batch (median of `reps` launches each, 0 = 5) and keeps the fastest per stage:

@@ -4,6 +4,7 @@ import ctypes
 import os
 import re
 import struct
+import subprocess
 
 import numpy as np
 import pytest
@@ -110,3 +111,13 @@ def test_generator_matches_committed_fixture_pixels():
     img = np.frombuffer(dec_rgb, np.uint8).reshape(512, 512, 3).astype(np.int32)
     assert len(dxt) == 512 * 512 // 2
     assert np.abs(img - rgb).mean() < 12
+
+
+def test_rans_units_cover_every_group_once():
+    """tests/cpp/ans_units_test.cpp (host code): the rANS work units of an image, with and
+    without short stream remainders paired in one work-group, cover each group of each
+    stream exactly once for every group-count combination up to 20 per stream."""
+    exe = os.path.join(ROOT, "tests", "cpp", "ans_units_test")
+    proc = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert proc.returncode == 0, proc.stdout + proc.stderr
+    assert "combinations checked" in proc.stdout

@@ -51,7 +51,8 @@ def test_uneven_batch_matches_oracle(device):
     (2, 8, 4, 2, False, False, False, True), (1, 8, 1, 4, True, False, False, True),
     (2, 1, 1, 0, True, False, False, True), (1, 2, 4, 0, True, True, False, True),
     (2, 4, 2, 2, False, False, True, True), (2, 8, 1, 4, False, True, True, True),
-    (2, 4, 1, 4, False, False, False, False), (1, 8, 1, 4, False, True, False, False)])
+    (2, 4, 1, 4, False, False, False, False), (1, 8, 1, 4, False, True, False, False),
+    (2, 2, 1, 4, False, False, False, False), (2, 2, 2, 2, False, False, False, True)])
 def test_launch_shapes_agree(device, gpw, waves, wwaves, awaves, separate, sep_tables, wide, sep_wavelet):
     files, dxts = encode.make_textures(MIXED[:4])
     blocks, pal = _capacity(files)
