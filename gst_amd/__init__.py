@@ -66,6 +66,11 @@ class Header(ctypes.Structure):
         return "Header(%s)" % ", ".join("%s=%d" % (n, getattr(self, n)) for n, _ in self._fields_)
 
 
+# rANS work-group forms of the 4-wave x 2-group shape (GST_ANS_FORM_*).
+ANS_FORMS = {0: "auto", 1: "paired", 2: "plain"}
+ANS_FORM_CODES = {v: k for k, v in ANS_FORMS.items()}
+
+
 class BatchConfig(ctypes.Structure):
     _fields_ = [
         ("capacity_images", ctypes.c_uint32),
@@ -84,7 +89,7 @@ class BatchConfig(ctypes.Structure):
         ("separate_table_stage", ctypes.c_uint32),
         ("assembly_wide", ctypes.c_uint32),
         ("separate_wavelet", ctypes.c_uint32),
-        ("ans_single_stream_units", ctypes.c_uint32),
+        ("ans_form", ctypes.c_uint32),
     ]
 
 
@@ -98,14 +103,14 @@ class TuneResult(ctypes.Structure):
         "ans_groups_per_wave", "ans_waves_per_group", "wavelet_waves_per_group", "assembly_waves_per_group",
         "ans_builds_tables", "assembly_wide", "grid_ans", "grid_indices", "grid_wavelet", "grid_assemble",
         "candidates")] + [("stage_ms", ctypes.c_float * 5), ("total_ms", ctypes.c_float),
-                          ("wavelet_fused", ctypes.c_uint32), ("ans_pairs_short", ctypes.c_uint32)]
+                          ("wavelet_fused", ctypes.c_uint32), ("ans_form", ctypes.c_uint32)]
 
     def as_dict(self):
         d = {name: getattr(self, name) for name, _ in self._fields_[:11]}
         d["stage_ms"] = _stage_times(self.stage_ms)
         d["total_ms"] = self.total_ms
         d["wavelet_fused"] = self.wavelet_fused
-        d["ans_pairs_short"] = self.ans_pairs_short
+        d["ans_form"] = ANS_FORMS.get(self.ans_form) if self.ans_form else None
         d["streams_per_wavefront"] = 32 * self.ans_groups_per_wave
         return d
 
@@ -339,14 +344,14 @@ class Batch:
     def __init__(self, capacity_images, capacity_blocks, capacity_palette=0, groups_per_wave=0,
                  waves_per_group=0, grids=None, rgb=False, wavelet_waves=0, assembly_waves=0,
                  separate_index_scan=False, separate_table_stage=False, assembly_wide=False,
-                 separate_wavelet=False, single_stream_units=False):
+                 separate_wavelet=False, ans_form="auto"):
         grids = grids or {}
         cfg = BatchConfig(capacity_images, capacity_blocks, capacity_palette, groups_per_wave,
                           waves_per_group, grids.get("ans_decode", 0), grids.get("inv_wavelet", 0),
                           grids.get("decode_indices", 0), grids.get("assemble", 0), 1 if rgb else 0,
                           wavelet_waves, assembly_waves, 1 if separate_index_scan else 0,
                           1 if separate_table_stage else 0, 1 if assembly_wide else 0,
-                          1 if separate_wavelet else 0, 1 if single_stream_units else 0)
+                          1 if separate_wavelet else 0, ANS_FORM_CODES[ans_form])
         self._h = _P()
         _check(lib().gst_batch_create(ctypes.byref(cfg), ctypes.byref(self._h)), "gst_batch_create")
 

@@ -148,9 +148,37 @@ StageConfig DefaultStageConfig(bool rgb) {
   s.fused_waves = rgb ? 0 : 4;
   s.tile_fused = rgb ? 0 : 1;
   s.ans_build_tables = 1;
-  s.ans_pair_short = 1;
+  s.ans_auto = 1;  // GST_ANS_FORM_AUTO: the caller sets ans_pair_short from AutoPairs
+  s.ans_pair_short = 0;
   s.rgb = rgb ? 1 : 0;
   return s;
+}
+
+// The rANS forms of the 4-wave x 2-group shape with tables built in LDS
+// (GST_ANS_FORM_*): PLAIN is AnsDecodeKernel<4, 2, true> (units of 8 groups of one
+// stream, 7 work-groups per CU); PAIRED and AUTO run AnsAutoKernel, which reads the
+// unit code of its plan: AUTO pairs short remainders only when the 8-group units would
+// not fit the device in one pass at 6 work-groups per CU.
+void SetAnsForm(StageConfig *s, uint32_t form, bool pair = true) {
+  s->ans_auto = (form == GST_ANS_FORM_PAIRED || form == GST_ANS_FORM_AUTO) ? 1 : 0;
+  s->ans_pair_short = (form == GST_ANS_FORM_PAIRED || (form == GST_ANS_FORM_AUTO && pair)) ? 1 : 0;
+}
+
+uint32_t AnsFormOf(const StageConfig &s) {
+  if (!s.ans_build_tables || s.ans_waves != 4 || s.ans_groups_per_wave != 2) return 0;
+  if (!s.ans_auto) return GST_ANS_FORM_PLAIN;
+  return s.ans_pair_short ? GST_ANS_FORM_PAIRED : GST_ANS_FORM_AUTO;
+}
+
+// Whether GST_ANS_FORM_AUTO pairs short remainders for these images: only when their
+// units of 8 groups would take more than one pass of 6 work-groups per CU (with the
+// 7th work-group of a CU a SIMD holds 7 decoding waves, and one pass is all that keeps
+// the per-stream chains overlapped).
+bool AutoPairs(const ImageDesc *descs, uint32_t n) {
+  uint64_t units = 0;
+  for (uint32_t i = 0; i < n; ++i) units += gst::AnsUnitsOf(descs[i], 8);
+  const uint64_t cus = static_cast<uint64_t>(G().cu_count > 0 ? G().cu_count : 256);
+  return units > 6 * cus;
 }
 
 // Groups per rANS unit, with gst::kAnsPairShort when short remainders pair up.
@@ -343,7 +371,7 @@ int DecodeStaging(const gst_header *hdrs, uint32_t n, hipStream_t s, const void 
   Scratch sc;
   if ((rc = TakeScratch(L.total, s, &sc)) != GST_OK) return rc;
 
-  const StageConfig cfg = DefaultStageConfig(rgb);
+  StageConfig cfg = DefaultStageConfig(rgb);
   std::vector<ImageDesc> descs(n);
   size_t in_off = 0, out_off = 0;
   for (uint32_t i = 0; i < n; ++i) {
@@ -352,6 +380,7 @@ int DecodeStaging(const gst_header *hdrs, uint32_t n, hipStream_t s, const void 
     in_off += StreamBytes(hdrs[i]);
     out_off += rgb ? 3ull * hdrs[i].width * hdrs[i].height : hdrs[i].width * hdrs[i].height / 2ull;
   }
+  SetAnsForm(&cfg, GST_ANS_FORM_AUTO, AutoPairs(descs.data(), n));
   uint32_t *tables = reinterpret_cast<uint32_t *>(sc.ptr + L.tables);
   uint32_t *status = reinterpret_cast<uint32_t *>(sc.ptr + L.status);
   uint32_t grids[gst::kNumStages];
@@ -359,6 +388,7 @@ int DecodeStaging(const gst_header *hdrs, uint32_t n, hipStream_t s, const void 
     KernelArgs a;
     memset(&a, 0, sizeof(a));
     a.single = descs[0];
+    a.ans_unit_code = AnsUnitCode(cfg);
     a.err = err;
     a.tables = tables;
     a.table_status = status;
@@ -568,13 +598,15 @@ int gst_load_compressed_dxt_noalloc(const gst_header *hdr, gst_stream stream, co
   c.planes = 0;  // not read: the wavelet runs inside the assembly
   c.total = off;
   if (c.total > gst::kNoAllocScratchBytes) return GST_ERR_TOO_LARGE;
-  const StageConfig cfg = DefaultStageConfig(false);
+  StageConfig cfg = DefaultStageConfig(false);
   const uint8_t *cmp = static_cast<const uint8_t *>(d_cmp);
   const size_t offsets_bytes = AlignUp(32, 512);  // codec/decoder.cpp:109-111 for one image
   KernelArgs a;
   memset(&a, 0, sizeof(a));
   FillDesc(*hdr, cmp + offsets_bytes, cmp + offsets_bytes + 2048, base, c, 0, static_cast<uint8_t *>(d_out),
            false, &a.single);
+  SetAnsForm(&cfg, GST_ANS_FORM_AUTO, AutoPairs(&a.single, 1));
+  a.ans_unit_code = AnsUnitCode(cfg);
   a.err = d_err_bits;
   uint32_t grids[gst::kNumStages];
   for (uint32_t st = 0; st < gst::kNumStages; ++st)
@@ -666,6 +698,7 @@ struct gst_batch {
   hipGraphExec_t exec = nullptr;
   uint32_t n_bound = 0;
   std::vector<ImageDesc> bound;  // descriptors of the bound images (for re-planning)
+  bool auto_form = false;        // GST_ANS_FORM_AUTO: the rANS form is chosen at each bind
 };
 
 namespace {
@@ -760,10 +793,14 @@ int gst_batch_create(const gst_batch_config *cfg, gst_batch **out) {
   b->stage.ans_waves = cfg->ans_waves_per_group ? cfg->ans_waves_per_group : 4;
   b->stage.wavelet_waves = cfg->wavelet_waves_per_group ? cfg->wavelet_waves_per_group : 1;
   b->stage.ans_build_tables = (cfg->separate_table_stage || b->stage.ans_waves < 4) ? 0 : 1;
-  b->stage.ans_pair_short = (b->stage.ans_build_tables && b->stage.ans_waves == 4 &&
-                             b->stage.ans_groups_per_wave == 2 && !cfg->ans_single_stream_units)
-                                ? 1
-                                : 0;
+  if (cfg->ans_form > GST_ANS_FORM_PLAIN) {
+    delete b;
+    return GST_ERR_INVALID_ARGUMENT;
+  }
+  const bool lds_4x2 = b->stage.ans_build_tables && b->stage.ans_waves == 4 && b->stage.ans_groups_per_wave == 2;
+  SetAnsForm(&b->stage, lds_4x2 ? cfg->ans_form : GST_ANS_FORM_PLAIN);
+  if (!lds_4x2) b->stage.ans_auto = b->stage.ans_pair_short = 0;
+  b->auto_form = lds_4x2 && cfg->ans_form == GST_ANS_FORM_AUTO;
   b->stage.fused_waves = (cfg->rgb_output || cfg->separate_index_scan)
                              ? 0
                              : (cfg->assembly_waves_per_group ? cfg->assembly_waves_per_group : 4);
@@ -828,6 +865,8 @@ int gst_batch_bind(gst_batch *b, const gst_header *hdrs, uint32_t n, const void 
              static_cast<uint8_t *>(d_outs[i]), b->stage.rgb != 0, &descs[i]);
     off += c.total;
   }
+  // The form travels in the plan, so a captured graph replays whichever one a bind picks.
+  if (b->auto_form) SetAnsForm(&b->stage, GST_ANS_FORM_AUTO, AutoPairs(descs.data(), n));
   b->bound = std::move(descs);
   uint32_t units[gst::kNumStages];
   const int rc = UploadPlan(b, s, units);
@@ -853,28 +892,29 @@ int gst_batch_autotune(gst_batch *b, gst_stream stream, uint32_t reps, gst_tune_
   // Tables as their own launch (one work-group per table); the rANS shapes that load
   // them are charged this time, the ones that build their table in LDS are not.
   best.ans_build_tables = 0;
-  best.ans_pair_short = 0;
+  best.ans_auto = best.ans_pair_short = 0;
   grids[gst::kStageTables] = 4 * b->n_bound;
   float t_tables = 0.f;
   if ((rc = TimeStage(b, gst::kStageTables, grids[gst::kStageTables], best, s, reps, &t_tables)) != GST_OK)
     return rc;
 
   // rANS: streams per wavefront (32 = one group per wave, 64 = two) x waves per
-  // work-group x table source (mode 0: loaded, 1: built in LDS, 2: built in LDS with
-  // short remainders paired) x persistent grid; each shape needs its own plan (units
-  // of W*GPW groups).
+  // work-group x table source (mode 0: loaded, 1: built in LDS; 2 and 3: AnsAutoKernel
+  // with and without short remainders paired, 4 waves x 2 groups) x persistent grid;
+  // each shape needs its own plan (units of W*GPW groups).
   static const uint32_t kAnsShapes[][2] = {{1, 2}, {1, 4}, {1, 8}, {2, 1}, {2, 2}, {2, 4}, {2, 8}};
   float best_ms = 1e30f, best_ans = 0.f;
   for (const auto &shape : kAnsShapes) {
-    for (uint32_t mode = 0; mode < 3; ++mode) {
-      const uint32_t build = mode >= 1 ? 1u : 0u, pair = mode == 2 ? 1u : 0u;
+    for (uint32_t mode = 0; mode < 4; ++mode) {
+      const uint32_t build = mode >= 1 ? 1u : 0u;
       if (build && shape[1] < 4) continue;  // the LDS build needs 256 threads
-      if (pair && !(shape[0] == 2 && shape[1] == 4)) continue;
+      if (mode >= 2 && !(shape[0] == 2 && shape[1] == 4)) continue;
       StageConfig c = best;
       c.ans_groups_per_wave = shape[0];
       c.ans_waves = shape[1];
       c.ans_build_tables = build;
-      c.ans_pair_short = pair;
+      c.ans_auto = mode >= 2 ? 1u : 0u;
+      c.ans_pair_short = mode == 2 ? 1u : 0u;
       b->stage = c;
       if ((rc = UploadPlan(b, s, units)) != GST_OK) return rc;
       const uint32_t cand[] = {cus, 2 * cus, 4 * cus, 8 * cus, units[gst::kStageAns]};
@@ -888,7 +928,8 @@ int gst_batch_autotune(gst_batch *b, gst_stream stream, uint32_t reps, gst_tune_
           best.ans_groups_per_wave = c.ans_groups_per_wave;
           best.ans_waves = c.ans_waves;
           best.ans_build_tables = build;
-          best.ans_pair_short = pair;
+          best.ans_auto = c.ans_auto;
+          best.ans_pair_short = c.ans_pair_short;
           grids[gst::kStageAns] = g;
         }
       }
@@ -1020,7 +1061,8 @@ int gst_batch_autotune(gst_batch *b, gst_stream stream, uint32_t reps, gst_tune_
   res.wavelet_waves_per_group = best.wavelet_waves;
   res.assembly_waves_per_group = best.fused_waves;
   res.ans_builds_tables = best.ans_build_tables;
-  res.ans_pairs_short = best.ans_pair_short;
+  res.ans_form = AnsFormOf(best);
+  b->auto_form = false;  // the tuned form stays for later binds
   res.assembly_wide = best.fused_wide;
   res.wavelet_fused = best.tile_fused;
   res.grid_ans = grids[gst::kStageAns];
@@ -1148,6 +1190,7 @@ int StageLaunch(uint32_t stage, const gst_header *hdr, const uint8_t *d_freqs, c
   cfg.fused_waves = 0;  // each stage is its own launch here
   cfg.tile_fused = 0;
   cfg.ans_build_tables = 0;
+  cfg.ans_auto = 0;
   cfg.ans_pair_short = 0;
   const uint32_t grid = gst::SinglePassGrid(stage, gst::StageUnitsOf(d, stage, AnsUnitCode(cfg)), cfg);
   GST_HIP(gst::LaunchStage(stage, a, grid, cfg, s));

@@ -182,6 +182,7 @@ __host__ __device__ inline size_t PlanBytesFor(uint32_t cap) { return PlanUsedBy
 struct KernelArgs {
   const PlanHeader *plan;
   uint32_t plan_cap;       // the plan's image capacity (fixes its layout)
+  uint32_t ans_unit_code;  // without a plan: the rANS unit code AnsAutoKernel decodes with
   uint32_t *err;
   uint32_t *tables;        // n_tables x 2048 packed entries
   uint32_t *table_status;  // n_tables: 1 = valid
@@ -196,6 +197,10 @@ struct StageConfig {
                                  // or 2 waves of two groups); the table stage launches nothing
   uint32_t ans_pair_short;       // 1: short stream remainders pair up in one work-group
                                  // (4 waves x 2 groups, tables built in LDS)
+  uint32_t ans_auto;             // 1: AnsAutoKernel (4 waves x 2 groups, tables built in LDS):
+                                 // ans_pair_short 1 runs the paired form, 0 the short-chain
+                                 // form (8-byte table and ring entries, one LDS round trip a
+                                 // step); the kernel takes the form from the unit code
   uint32_t wavelet_waves;        // waves per wavelet work-group (1, 2, 4); a wave takes 2 units
   uint32_t fused_waves;          // 0: separate index scan and assembly; else waves per
                                  // work-group of the fused scan + DXT1 assembly (1, 2, 4)

@@ -251,6 +251,14 @@ __global__ __launch_bounds__(256) void BuildTableKernel(KernelArgs a) {
 // Index-stream groups also emit
 // the sum of (symbol - 128) per group and per lane, the totals the index scan
 // starts from.
+// LDS written by some lanes of a wave and read by others: the wave's LDS operations
+// complete in order, so only the compiler needs a fence.
+__device__ __forceinline__ void WaveSync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 constexpr uint32_t kChunkWords = 256;
 constexpr uint32_t kRingWords = 2 * kChunkWords;
 constexpr uint32_t kRefillSteps = kChunkWords / kLanesPerGroup;  // 8
@@ -611,22 +619,35 @@ __device__ __forceinline__ bool BuildTableInLds(uint32_t f, uint32_t *tab, uint3
 // offsets -> lane states and ring chunks), all from HBM in a full pipeline; the
 // frequencies go out with the offsets and the states and chunks before the table is
 // built, so the LDS build overlaps their latency.
-template <uint32_t WAVES, uint32_t GPW, bool kBuild, bool kPair = false>
-// With 4 or 8 waves LDS allows 7 waves per SIMD (6 with the second table); hold the
-// VGPRs to that (at most 72, or 80).
-__global__ __launch_bounds__(WAVES * 64, kPair ? 6 : WAVES >= 4 ? 7 : 1)
-void AnsDecodeKernel(KernelArgs a) {
+// LDS of the rANS kernels beside their tables and rings.
+template <uint32_t WAVES>
+struct AnsLdsSmall {
+  uint32_t build_wsum[4];
+  uint32_t build_before[WAVES];
+  uint32_t touch_sink[64];
+};
+
+// The LDS of AnsDecodeUnits: tables (one, two with kPair) and the half-waves' rings.
+// The table build's scratch lives in the rings: every wave has left the previous
+// unit's ring (barrier) before the build, and the rings are filled after it.
+template <uint32_t WAVES, bool kPair>
+struct AnsLds {
+  uint32_t tab[kPair ? 2 : 1][kTableSize];
+  uint16_t ring[WAVES][2][kRingWords];
+};
+
+template <uint32_t WAVES, uint32_t GPW, bool kBuild, bool kPair>
+__device__ __forceinline__ void AnsDecodeUnits(const KernelArgs &a, AnsLds<WAVES, kPair> &L,
+                                               AnsLdsSmall<WAVES> &S) {
   static_assert(!kPair || (kBuild && WAVES % 2 == 0), "paired units build their tables in LDS");
-  __shared__ __align__(16) uint32_t tab[kPair ? 2 : 1][kTableSize];
-  __shared__ __align__(16) uint16_t ring[WAVES][2][kRingWords];
-  // The table build's scratch lives in the rings: every wave has left the previous
-  // unit's ring (barrier) before the build, and the rings are filled after it.
   static_assert(WAVES * 2 * kRingWords * 2 >= 256 * 4 + 256, "build scratch fits the rings");
+  auto &tab = L.tab;
+  auto &ring = L.ring;
   uint32_t *const build_fc = reinterpret_cast<uint32_t *>(&ring[0][0][0]);
   uint8_t *const build_sym = reinterpret_cast<uint8_t *>(build_fc + 256);
-  __shared__ uint32_t build_wsum[4];
-  __shared__ uint32_t build_before[WAVES];
-  __shared__ uint32_t touch_sink[64];
+  uint32_t *const build_wsum = S.build_wsum;
+  uint32_t *const build_before = S.build_before;
+  uint32_t *const touch_sink = S.touch_sink;
   constexpr uint32_t kGroupsPerUnit = WAVES * GPW;
   constexpr uint32_t kUnitCode = kGroupsPerUnit | (kPair ? kAnsPairShort : 0u);
   const uint32_t lane = threadIdx.x & 63, wave = Uniform(threadIdx.x >> 6);
@@ -713,6 +734,38 @@ void AnsDecodeKernel(KernelArgs a) {
       DecodeHalfGroups<false, GPW>(tw, ring[wave][0], ring_half, touch_sink, rs, chunks, lane, active, x, pA, pB,
                                    loA, loB, outp, nullptr, nullptr, nullptr, a.err);
   }
+}
+
+template <uint32_t WAVES, uint32_t GPW, bool kBuild, bool kPair = false>
+// With 4 or 8 waves LDS allows 7 waves per SIMD (6 with the second table); hold the
+// VGPRs to that (at most 72, or 80).
+__global__ __launch_bounds__(WAVES * 64, kPair ? 6 : WAVES >= 4 ? 7 : 1)
+void AnsDecodeKernel(KernelArgs a) {
+  __shared__ __align__(16) AnsLds<WAVES, kPair> lds;
+  __shared__ AnsLdsSmall<WAVES> small;
+  AnsDecodeUnits<WAVES, GPW, kBuild, kPair>(a, lds, small);
+}
+
+// The rANS stage whose work-unit form is data: the plan's unit code (or, for a single
+// image, the one in the arguments) selects units of 8 groups of one stream or the
+// paired form (short remainders share a work-group), both AnsDecodeUnits<4, 2> with
+// tables built in LDS.  The two share the LDS of the larger, the paired form's two
+// tables (6 work-groups per CU), and a captured graph of this launch replays either,
+// so a batch can change forms from bind to bind.
+constexpr uint32_t kAutoWaves = 4;
+union AnsAutoLds {
+  AnsLds<kAutoWaves, false> single;
+  AnsLds<kAutoWaves, true> paired;
+};
+
+__global__ __launch_bounds__(kAutoWaves * 64, 6) void AnsAutoKernel(KernelArgs a) {
+  __shared__ __align__(16) AnsAutoLds lds;
+  __shared__ AnsLdsSmall<kAutoWaves> small;
+  const uint32_t code = a.plan ? Uniform(Gbl<const uint32_t>(&a.plan->ans_unit_code)[0]) : a.ans_unit_code;
+  if (code & kAnsPairShort)
+    AnsDecodeUnits<kAutoWaves, 2, true, true>(a, lds.paired, small);
+  else
+    AnsDecodeUnits<kAutoWaves, 2, true, false>(a, lds.single, small);
 }
 
 // ---------------------------------------------------------------------------
@@ -865,12 +918,6 @@ __device__ __forceinline__ s16x2 TruncDiv4(s16x2 s) {
 __device__ __forceinline__ s16x2 TruncDiv2(s16x2 s) {
   const u16x2 sign = __builtin_bit_cast(u16x2, s) >> static_cast<u16x2>(15);  // 0 or 1
   return (s + __builtin_bit_cast(s16x2, sign)) >> static_cast<s16x2>(1);
-}
-
-__device__ __forceinline__ void WaveSync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 __host__ __device__ constexpr int Mirror(int idx, int range) {
@@ -1504,10 +1551,13 @@ hipError_t LaunchStage(uint32_t stage, const KernelArgs &args, uint32_t grid,
       return hipGetLastError();
     case kStageAns: {
       const uint32_t w = cfg.ans_waves, g = cfg.ans_groups_per_wave;
-      if (cfg.ans_pair_short) {
-        if (cfg.ans_build_tables && g == 2 && w == 4) return LaunchAns<4, 2, true, true>(args, grid, stream);
-        return hipErrorInvalidValue;
+      if (cfg.ans_auto) {
+        // The form travels in the plan (or in args.ans_unit_code for a single image).
+        if (!cfg.ans_build_tables || g != 2 || w != kAutoWaves) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(AnsAutoKernel, dim3(grid), dim3(kAutoWaves * 64), 0, stream, args);
+        return hipGetLastError();
       }
+      if (cfg.ans_pair_short) return hipErrorInvalidValue;  // paired units run in AnsAutoKernel
       if (cfg.ans_build_tables) {
         if (g == 2 && w == 4) return LaunchAns<4, 2, true>(args, grid, stream);
         if (g == 2 && w == 8) return LaunchAns<8, 2, true>(args, grid, stream);

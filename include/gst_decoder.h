@@ -208,12 +208,18 @@ typedef struct gst_batch_config {
                                        (one scan per 1024 blocks) instead of 4 x 4 */
   uint32_t separate_wavelet;        /* 1: the inverse wavelet as its own launch; 0: it runs
                                        inside the DXT1 assembly (planes stay in LDS) */
-  uint32_t ans_single_stream_units; /* 1: every rANS work-group decodes groups of one stream;
-                                       0: with 4 waves x 2 groups and tables built in LDS, a
-                                       stream's remainder of at most 4 groups shares a
-                                       work-group with another stream's (fewer work-groups, at
-                                       most 6 per CU) */
+  uint32_t ans_form;                /* rANS kernel with 4 waves x 2 groups and tables built in
+                                       LDS (the default shape): GST_ANS_FORM_* */
 } gst_batch_config;
+
+/* rANS work-group forms (gst_batch_config.ans_form, gst_tune_result.ans_form). */
+#define GST_ANS_FORM_AUTO 0   /* chosen at each bind: work-groups of 8 groups of one stream
+                                 while they fit the device in one pass at 6 per CU, else
+                                 PAIRED (one captured graph replays both) */
+#define GST_ANS_FORM_PAIRED 1 /* a stream's remainder of at most 4 groups shares a
+                                 work-group with another stream's: fewer work-groups, at
+                                 most 6 per CU */
+#define GST_ANS_FORM_PLAIN 2  /* work-groups of 8 groups of one stream, 7 per CU */
 
 int gst_batch_create(const gst_batch_config *cfg, gst_batch **out);
 /* Binds n images: headers, device pointers to each .gtc file and each output.
@@ -249,7 +255,10 @@ typedef struct gst_tune_result {
                                         stage the chosen shapes fold into another launch */
   float total_ms;
   uint32_t wavelet_fused;            /* 1: wavelet + index scan + DXT1 assembly in one kernel */
-  uint32_t ans_pairs_short;          /* 1: short stream remainders share rANS work-groups */
+  uint32_t ans_form;                 /* GST_ANS_FORM_PAIRED, _PLAIN or _AUTO (8-group units in
+                                        the kernel that also runs PAIRED) of the chosen rANS
+                                        shape when it is 4 waves x 2 groups with tables built
+                                        in LDS; else 0 */
 } gst_tune_result;
 /* No reference counterpart.  Times every launch shape of every stage on the bound
  * batch (median of `reps` launches each, 0 = 5) and keeps the fastest per stage:

@@ -275,3 +275,106 @@ def test_two_plans_in_flight(device):
     finally:
         for p in plans:
             p.close()
+
+
+def _garbled(gtc, stream, seed):
+    """ref_c0 with 200 bytes of noise in the renormalisation words of one stream (the
+    group offsets stay valid): a different bitstream that still decodes everywhere."""
+    h = gst_amd.ParseHeader(gtc)
+    starts = {"y": 28 + 2048 + 4 * (2 * h.num_blocks // 8192),
+              "chroma": 28 + 2048 + h.y_cmp_sz + 4 * (4 * h.num_blocks // 8192),
+              "palette": 28 + 2048 + h.y_cmp_sz + h.chroma_cmp_sz + 4 * (h.palette_bytes // 8192),
+              "indices": 28 + 2048 + h.y_cmp_sz + h.chroma_cmp_sz + h.palette_sz + 4 * (h.num_blocks // 8192)}
+    start = starts[stream]
+    b = bytearray(gtc)
+    b[start:start + 200] = np.random.default_rng(seed).integers(0, 256, 200, dtype=np.uint8).tobytes()
+    return bytes(b)
+
+
+@pytest.mark.parametrize("form", ["auto", "paired", "plain"])
+def test_ans_forms_match_oracle(device, form):
+    """Every rANS work-group form of the 4-wave x 2-group shape on an uneven batch, with
+    reference files and garbled streams (noise in the words: each form must decode the
+    bits exactly as the oracle does, including palette indices that leave the palette)."""
+    files, dxts = encode.make_textures(MIXED)
+    files += [load_golden("ref_test1")["gtc"].tobytes()]
+    dxts += [load_golden("ref_test1")["dxt"].tobytes()]
+    c0 = load_golden("ref_c0")["gtc"].tobytes()
+    bad_palette = False
+    # Seeds whose noise keeps every group's words inside the group (the oracle decodes
+    # them without a stream error).
+    for stream, seed in (("y", 10), ("chroma", 11), ("palette", 13)):
+        g = _garbled(c0, stream, seed)
+        rc, expected, _ = go.decode_dxt(g, nthreads=8)
+        assert rc >= 0
+        bad_palette |= rc > 0
+        files.append(g)
+        dxts.append(expected)
+    blocks, pal = _capacity(files)
+    b = gst_amd.Batch(len(files), blocks, pal, ans_form=form)
+    try:
+        _, _, d_outs = _bind(device, b, files)  # outputs poisoned
+        b.decode(device.stream)
+        for i, (d, out) in enumerate(zip(dxts, d_outs)):
+            assert device.download(out) == d, "image %d" % i
+        assert b.error_bits(device.stream) == (0x4 if bad_palette else 0)
+    finally:
+        b.close()
+
+
+@pytest.mark.parametrize("form", ["auto", "paired"])
+def test_ans_forms_report_bad_frequencies_alike(device, form):
+    """A frequency table that does not sum to 2048 raises the frequency bit in every form,
+    and every form decodes it with the same identity table (same output bytes)."""
+    c0 = load_golden("ref_c0")["gtc"].tobytes()
+    b = bytearray(c0)
+    b[28:30] = (int.from_bytes(b[28:30], "little") + 3).to_bytes(2, "little")
+    files = [bytes(b), encode.make_textures([(310, 512, 512)])[0][0]]
+    blocks, pal = _capacity(files)
+    outs = {}
+    for f in ("plain", form):
+        batch = gst_amd.Batch(len(files), blocks, pal, ans_form=f)
+        try:
+            _, _, d_outs = _bind(device, batch, files)
+            batch.decode(device.stream)
+            outs[f] = [device.download(o) for o in d_outs]
+            assert batch.error_bits(device.stream) & 0x2
+        finally:
+            batch.close()
+    assert outs[form] == outs["plain"]
+
+
+def test_auto_form_switches_under_one_captured_graph(device):
+    """GST_ANS_FORM_AUTO decodes a small batch in work-groups of 8 groups of one stream and
+    pairs short remainders once those work-groups no longer fit the device in one pass;
+    the rANS launch reads its form from the plan, so one captured graph replays both,
+    bit-exact."""
+    torch = device.torch
+    (big,), (big_dxt,) = encode.make_textures([(320, 4096, 4096)])
+    (small,), (small_dxt,) = encode.make_textures([(321, 512, 512)])
+    h = gst_amd.ParseHeader(big)
+    per_big = sum((g + 7) // 8 for g in (2 * h.num_blocks // 8192, 4 * h.num_blocks // 8192,
+                                         h.palette_bytes // 8192, h.num_blocks // 8192))
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    n_big = 6 * cus // per_big + 1  # past one pass of 6 work-groups per CU
+    assert n_big * per_big > 6 * cus
+    b = gst_amd.Batch(n_big + 1, (n_big + 1) * h.num_blocks, (n_big + 1) * h.palette_bytes)
+    try:
+        _, keep_s, outs_s = _bind(device, b, [small])
+        b.capture_graph(device.stream)
+        b.launch_graph(device.stream)
+        assert device.download(outs_s[0]) == small_dxt
+        _, keep_b, outs_b = _bind(device, b, [big] * n_big + [small])
+        for rep in range(2):
+            for o in outs_b:
+                device.poison(o)
+            b.launch_graph(device.stream)
+            device.sync()
+            got = [device.download(o) for o in outs_b]
+            assert got[:-1] == [big_dxt] * n_big and got[-1] == small_dxt, "replay %d" % rep
+        _, keep_s2, outs_s2 = _bind(device, b, [small])
+        b.launch_graph(device.stream)
+        assert device.download(outs_s2[0]) == small_dxt
+        assert b.error_bits(device.stream) == 0
+    finally:
+        b.close()
