@@ -628,6 +628,12 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": measured_traffic(dom, len(hdrs), args.images),
         },
+        "roofline_by_kernel": {
+            k: {"achieved": round(sb[k] / (v * 1e-3) / 1e9, 1),
+                "frac": round(sb[k] / (v * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                "algorithmic_bytes": sb[k],
+                "traffic": measured_traffic(k, len(hdrs), args.images)}
+            for k, v in launched.items() if sb[k]},
         "pipeline_roofline": {
             "B_pipe_bytes": b_pipe,
             "B_pipe_frac": round(b_pipe / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4),
