@@ -107,7 +107,9 @@ struct Req {
   uint64_t dev_in_off = 0, dev_in_bytes = 0;      // arena: input (host inputs only)
   uint64_t dev_out_off = 0, out_bytes = 0;        // arena: output
   uint64_t in_bytes = 0;  // device input layout: the file, or the reference staging layout (plain)
-  uint64_t copy_seq = 0;  // the H2D copy that brings the input (0: none)
+  uint64_t copy_seq = 0;  // the copy-stream H2D copy that brings the input (0: none)
+  bool copy_pending = false;  // admitted, input not copied yet: the launch copies it
+  bool in_batch = false;      // taken into a batch (scheduler thread)
   bool admitted = false;
   Clock::time_point t_arrival, t_launch, t_dev, t_host;
   int status = GST_OK;
@@ -237,31 +239,37 @@ void PublishLocked(gst_server &s, const std::shared_ptr<Req> &r) {
 // Copies the inputs of newly admitted host requests to the arena on the copy stream,
 // merging requests whose staging and arena ranges are both contiguous, and marks the
 // group with one event (scheduler thread only).
-int IssueCopies(gst_server &s, const std::vector<std::shared_ptr<Req>> &fresh) {
-  if (fresh.empty() || s.cfg.dry_run) return GST_OK;
+// Coalesced H2D copies of the inputs of `reqs` (adjacent in both the pinned staging and
+// the arena: one copy) on `stream`.  On the copy stream (`event`) the group gets an event
+// that the launching slot's stream waits for; on a slot's own stream the decode that
+// follows is ordered after them anyway.
+int IssueCopies(gst_server &s, const std::vector<std::shared_ptr<Req>> &reqs, hipStream_t stream, bool event) {
+  if (reqs.empty() || s.cfg.dry_run) return GST_OK;
   size_t i = 0;
   uint64_t bytes = 0, copies = 0;
-  while (i < fresh.size()) {
-    const Req &a = *fresh[i];
+  while (i < reqs.size()) {
+    const Req &a = *reqs[i];
     uint64_t h0 = a.host_in_off, d0 = a.dev_in_off, n = a.in_bytes;
     size_t j = i + 1;
-    for (; j < fresh.size(); ++j) {
-      const Req &b = *fresh[j];
+    for (; j < reqs.size(); ++j) {
+      const Req &b = *reqs[j];
       const uint64_t gap = AlignUp(h0 + n) - (h0 + n);
       if (b.host_in_off != h0 + n + gap || b.dev_in_off != d0 + n + gap) break;
       n += gap + b.in_bytes;
     }
-    const hipError_t e =
-        hipMemcpyAsync(s.d_arena + d0, s.h_staging + h0, n, hipMemcpyHostToDevice, s.copy_stream);
+    const hipError_t e = hipMemcpyAsync(s.d_arena + d0, s.h_staging + h0, n, hipMemcpyHostToDevice, stream);
     if (e != hipSuccess) return HipErr(e);
     bytes += n;
     ++copies;
     i = j;
   }
-  const uint64_t seq = ++s.copy_seq;
-  const hipError_t e = hipEventRecord(s.copy_ev[seq % kCopyEvents], s.copy_stream);
-  if (e != hipSuccess) return HipErr(e);
-  for (auto &r : fresh) r->copy_seq = seq;
+  for (auto &r : reqs) r->copy_pending = false;
+  if (event) {
+    const uint64_t seq = ++s.copy_seq;
+    const hipError_t e = hipEventRecord(s.copy_ev[seq % kCopyEvents], stream);
+    if (e != hipSuccess) return HipErr(e);
+    for (auto &r : reqs) r->copy_seq = seq;
+  }
   std::lock_guard<std::mutex> lk(s.mu);
   s.st.h2d_bytes += bytes;
   s.st.h2d_copies += copies;
@@ -272,10 +280,12 @@ int IssueCopies(gst_server &s, const std::vector<std::shared_ptr<Req>> &fresh) {
 int LaunchBatch(gst_server &s, Slot &slot, std::vector<std::shared_ptr<Req>> &&reqs) {
   const auto now = Clock::now();
   uint64_t wait_seq = 0;
+  std::vector<std::shared_ptr<Req>> to_copy;  // admitted in this round: copied on the slot's stream
   for (auto &r : reqs) {
     r->t_launch = now;
     r->batch_images = static_cast<uint32_t>(reqs.size());
     wait_seq = std::max(wait_seq, r->copy_seq);
+    if (r->copy_pending) to_copy.push_back(r);
   }
   slot.reqs = std::move(reqs);
   slot.dev_seen = false;
@@ -295,6 +305,10 @@ int LaunchBatch(gst_server &s, Slot &slot, std::vector<std::shared_ptr<Req>> &&r
   hipError_t e = hipSuccess;
   if (wait_seq) {
     if ((e = hipStreamWaitEvent(st, s.copy_ev[wait_seq % kCopyEvents], 0)) != hipSuccess) return HipErr(e);
+  }
+  if (!to_copy.empty()) {
+    const int rc = IssueCopies(s, to_copy, st, false);
+    if (rc != GST_OK) return rc;
   }
   if (s.cfg.poison_outputs) {
     for (auto &r : slot.reqs)
@@ -320,7 +334,7 @@ int LaunchBatch(gst_server &s, Slot &slot, std::vector<std::shared_ptr<Req>> &&r
     }
     rc = gst_batch_bind(slot.batch, hdrs.data(), static_cast<uint32_t>(hdrs.size()), files.data(),
                         outs.data(), st);
-    if (!s.cfg.direct_launch) {
+    if (s.cfg.graph_launch) {
       if (rc == GST_OK && !slot.graph_ready) {
         rc = gst_batch_capture_graph(slot.batch, st);
         slot.graph_ready = rc == GST_OK;
@@ -433,6 +447,7 @@ void SchedulerLoop(gst_server *sp) {
   gst_server &s = *sp;
   if (!s.cfg.dry_run) (void)hipSetDevice(0);
   const uint64_t max_blocks = s.cfg.max_batch_blocks;
+  auto last_work = Clock::now();
   for (;;) {
     bool progress = false;
     for (auto &slot : s.slots) progress |= PollSlot(s, slot);
@@ -459,6 +474,7 @@ void SchedulerLoop(gst_server *sp) {
         if (copy) {
           r->dev_in_off = in_off;
           r->dev_in_bytes = r->in_bytes;
+          r->copy_pending = true;
           fresh.push_back(r);
         }
         r->admitted = true;
@@ -467,8 +483,35 @@ void SchedulerLoop(gst_server *sp) {
         s.st.peak_arena_bytes = std::max<uint64_t>(s.st.peak_arena_bytes, s.dev_alloc.used());
       }
     }
+    {
+      std::lock_guard<std::mutex> lk(s.mu);
+      for (auto &slot : s.slots) {
+        if (slot.busy || s.admitted.empty()) continue;
+        std::vector<std::shared_ptr<Req>> b;
+        uint64_t blocks = 0;
+        const uint32_t max_images = s.cfg.plain ? 1u : s.cfg.max_batch_images;
+        while (!s.admitted.empty() && b.size() < max_images) {
+          auto &r = s.admitted.front();
+          const uint64_t nv = static_cast<uint64_t>(r->hdr.width) * r->hdr.height / 16;
+          if (!b.empty() && blocks + nv > max_blocks) break;
+          blocks += nv;
+          r->in_batch = true;
+          b.push_back(r);
+          s.admitted.pop_front();
+        }
+        batches.push_back(std::move(b));
+        targets.push_back(&slot);
+        slot.busy = true;  // reserved until LaunchBatch runs below
+      }
+    }
+    // Inputs admitted now that no free slot takes in this round go ahead on the copy
+    // stream, overlapping the decodes in flight; the others are copied on their slot's
+    // stream right before their decode (no cross-stream wait on the launch path).
+    fresh.erase(std::remove_if(fresh.begin(), fresh.end(),
+                               [](const std::shared_ptr<Req> &r) { return r->in_batch; }),
+                fresh.end());
     if (!fresh.empty()) {
-      const int rc = IssueCopies(s, fresh);
+      const int rc = IssueCopies(s, fresh, s.copy_stream, true);
       if (rc != GST_OK) {
         // The copies may be partly enqueued: drain the copy stream, then fail the
         // requests that were to use them.
@@ -487,26 +530,6 @@ void SchedulerLoop(gst_server *sp) {
       }
       progress = true;
     }
-    {
-      std::lock_guard<std::mutex> lk(s.mu);
-      for (auto &slot : s.slots) {
-        if (slot.busy || s.admitted.empty()) continue;
-        std::vector<std::shared_ptr<Req>> b;
-        uint64_t blocks = 0;
-        const uint32_t max_images = s.cfg.plain ? 1u : s.cfg.max_batch_images;
-        while (!s.admitted.empty() && b.size() < max_images) {
-          auto &r = s.admitted.front();
-          const uint64_t nv = static_cast<uint64_t>(r->hdr.width) * r->hdr.height / 16;
-          if (!b.empty() && blocks + nv > max_blocks) break;
-          blocks += nv;
-          b.push_back(r);
-          s.admitted.pop_front();
-        }
-        batches.push_back(std::move(b));
-        targets.push_back(&slot);
-        slot.busy = true;  // reserved until LaunchBatch runs below
-      }
-    }
     for (size_t i = 0; i < batches.size(); ++i) {
       Slot &slot = *targets[i];
       const auto t0 = Clock::now();
@@ -521,6 +544,7 @@ void SchedulerLoop(gst_server *sp) {
     }
     bool busy = false;
     for (auto &slot : s.slots) busy |= slot.busy;
+    if (progress || busy) last_work = Clock::now();
     if (!progress) {
       std::unique_lock<std::mutex> lk(s.mu);
       if (s.stop && !busy && s.admitted.empty()) {
@@ -538,7 +562,12 @@ void SchedulerLoop(gst_server *sp) {
         break;
       }
       if (!busy && s.arrivals.empty() && s.admitted.empty()) {
-        s.cv_work.wait_for(lk, std::chrono::milliseconds(2));
+        if (Micros(last_work, Clock::now()) < s.cfg.idle_spin_us) {
+          lk.unlock();
+          std::this_thread::yield();  // still polling: an arrival starts without a wake-up
+        } else {
+          s.cv_work.wait_for(lk, std::chrono::milliseconds(2));
+        }
       } else if (!busy) {
         // Waiting for memory that only a release can free.
         s.cv_work.wait_for(lk, std::chrono::microseconds(200));

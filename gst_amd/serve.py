@@ -19,9 +19,10 @@ class ServerConfig(ctypes.Structure):
         ("dry_run", ctypes.c_uint32),
         ("ans_groups_per_wave", ctypes.c_uint32),
         ("ans_waves_per_group", ctypes.c_uint32),
-        ("direct_launch", ctypes.c_uint32),
+        ("graph_launch", ctypes.c_uint32),
         ("poison_outputs", ctypes.c_uint32),
         ("fail_launch_every", ctypes.c_uint32),
+        ("idle_spin_us", ctypes.c_uint32),
     ]
 
 
@@ -116,11 +117,12 @@ def _lib():
 class Server:
     def __init__(self, hbm_budget_bytes, host_staging_bytes=0, slots=0, max_batch_images=0,
                  max_batch_blocks=0, copy_to_host=False, auto_release=True, plain=False, dry_run=False,
-                 groups_per_wave=0, waves_per_group=0, direct_launch=False, poison_outputs=False,
-                 fail_launch_every=0):
+                 groups_per_wave=0, waves_per_group=0, graph_launch=False, poison_outputs=False,
+                 fail_launch_every=0, idle_spin_us=0):
         cfg = ServerConfig(hbm_budget_bytes, host_staging_bytes, slots, max_batch_images, max_batch_blocks,
                            int(copy_to_host), int(auto_release), int(plain), int(dry_run), groups_per_wave,
-                           waves_per_group, int(direct_launch), int(poison_outputs), fail_launch_every)
+                           waves_per_group, int(graph_launch), int(poison_outputs), fail_launch_every,
+                           idle_spin_us)
         self._h = ctypes.c_void_p()
         _check(_lib().gst_server_create(ctypes.byref(cfg), ctypes.byref(self._h)), "gst_server_create")
 

@@ -5,12 +5,13 @@
  * fixed pages of equal-size images (demo/photos_sf.cpp:688-830).  Requests are
  * admitted against an HBM budget: each admitted request holds its input (unless it
  * is already device-resident) and its output in a device arena until it is released.
- * Inputs submitted from the host go to pinned staging at submit; at admission the
- * scheduler copies them to the arena on a copy stream of its own, contiguous
- * requests in one copy, so copies overlap the decodes in flight.  A scheduler thread
+ * Inputs submitted from the host go to pinned staging at submit.  A scheduler thread
  * forms a batch from every admitted request as soon as one of `slots` decode slots
- * is free and launches it (one gst_batch plan per slot, replayed from a captured
- * hipGraph, each slot on its own stream, after the copy that brought its last input),
+ * is free and launches it (one gst_batch plan per slot, each slot on its own stream):
+ * inputs admitted in that same round are copied to the arena on the slot's stream right
+ * before their decode, and inputs that find no free slot go ahead on a copy stream of
+ * the scheduler's (contiguous requests in one copy), overlapping the decodes in flight,
+ * with the slot waiting for that copy when their batch launches;
  * so requests that arrive while other batches run are admitted as memory frees up and
  * their work overlaps on the device.  Outputs stay device-resident; an optional copy
  * back to pinned host memory is timed separately, and an optional expected output per
@@ -42,13 +43,16 @@ typedef struct gst_server_config {
   uint32_t dry_run;             /* 1: no GPU work; decodes complete at once (scheduler tests) */
   uint32_t ans_groups_per_wave; /* batch plan launch shape (0: defaults) */
   uint32_t ans_waves_per_group;
-  uint32_t direct_launch;       /* 0: replay each slot's captured hipGraph (one call per
-                                   batch for the scheduler thread); 1: launch the stage kernels
-                                   on the slot's stream (lower p50 at light load, more host
-                                   time per batch) */
+  uint32_t graph_launch;        /* 1: replay each slot's captured hipGraph; 0: launch the
+                                   stage kernels on the slot's stream (on MI355X both the
+                                   scheduler's host time per batch and the decode are shorter
+                                   this way: 34 against 53 us and about 3 us) */
   uint32_t poison_outputs;      /* 1: fill each request's output with 0xA5 before its decode, so
                                    a decode that writes nothing cannot pass a check */
   uint32_t fail_launch_every;   /* testing, dry_run only: every n-th batch launch fails (0: never) */
+  uint32_t idle_spin_us;        /* an idle scheduler keeps polling for this long after its last
+                                   work before it sleeps on the arrival condition (0: sleeps at
+                                   once): trades a host core for the wake-up latency */
 } gst_server_config;
 
 /* One request.  Either `file` holds the whole .gtc on the host (copied to pinned

@@ -169,8 +169,8 @@ def served_corpus():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("direct_launch", [False, True])
-def test_served_results_equal_alone_and_in_the_mix(device, served_corpus, direct_launch):
+@pytest.mark.parametrize("graph_launch", [False, True])
+def test_served_results_equal_alone_and_in_the_mix(device, served_corpus, graph_launch):
     """Each request decoded alone (one at a time) and in a concurrent stream of
     arrivals gives the same bit-exact blocks, with graph replays or stream launches.
     Outputs are poisoned before every decode, and the in-mix run also compares each
@@ -178,7 +178,7 @@ def test_served_results_equal_alone_and_in_the_mix(device, served_corpus, direct
     files, dxts = served_corpus
     download = _hip_download()
     alone = {}
-    with Server(1 << 30, auto_release=False, direct_launch=direct_launch, poison_outputs=True) as s:
+    with Server(1 << 30, auto_release=False, graph_launch=graph_launch, poison_outputs=True) as s:
         for i, f in enumerate(files):
             s.submit(f, tag=i)
             s.drain()
@@ -189,7 +189,7 @@ def test_served_results_equal_alone_and_in_the_mix(device, served_corpus, direct
     assert [alone[i] for i in range(len(files))] == dxts
 
     d_expected = [device.upload(d) for d in dxts]
-    with Server(1 << 30, auto_release=False, slots=3, direct_launch=direct_launch, poison_outputs=True) as s:
+    with Server(1 << 30, auto_release=False, slots=3, graph_launch=graph_launch, poison_outputs=True) as s:
         order = list(range(len(files))) * 4
         np.random.default_rng(0).shuffle(order)
         for i in order:
