@@ -317,12 +317,14 @@ int LaunchBatch(gst_server &s, Slot &slot, std::vector<std::shared_ptr<Req>> &&r
   if ((e = hipEventRecord(slot.ev_start, st)) != hipSuccess) return HipErr(e);
   int rc = GST_OK;
   if (s.cfg.plain) {
-    // One request per batch; its own error bits follow it to pinned memory.
+    // One request per batch; its own error bits follow it to pinned memory (after the
+    // output-ready event: they are read at the batch's end).
     for (auto &r : slot.reqs) {
       rc = gst_load_compressed_dxt(&r->hdr, st, InputPtr(s, *r), OutputPtr(s, *r), 0, nullptr, nullptr);
-      if (rc == GST_OK) rc = gst_read_error_bits_async(st, slot.err_host);
       if (rc != GST_OK) return rc;
     }
+    if ((e = hipEventRecord(slot.ev_dev, st)) != hipSuccess) return HipErr(e);
+    if ((rc = gst_read_error_bits_async(st, slot.err_host)) != GST_OK) return rc;
   } else {
     std::vector<gst_header> hdrs;
     std::vector<const void *> files;
@@ -343,10 +345,12 @@ int LaunchBatch(gst_server &s, Slot &slot, std::vector<std::shared_ptr<Req>> &&r
     } else if (rc == GST_OK) {
       rc = gst_batch_decode(slot.batch, st);
     }
-    if (rc == GST_OK) rc = gst_batch_error_bits_async(slot.batch, st, slot.err_host);
     if (rc != GST_OK) return rc;
+    // Output ready in HBM here; the error bits' copy back (read at the batch's end) and
+    // the optional comparison come after.
+    if ((e = hipEventRecord(slot.ev_dev, st)) != hipSuccess) return HipErr(e);
+    if ((rc = gst_batch_error_bits_async(slot.batch, st, slot.err_host)) != GST_OK) return rc;
   }
-  if ((e = hipEventRecord(slot.ev_dev, st)) != hipSuccess) return HipErr(e);
   // Comparison with the expected outputs, after the point the device latency stops.
   gst::CompareArgs ca;
   memset(&ca, 0, sizeof(ca));
