@@ -156,7 +156,7 @@ def check_outputs(gpu, d_out, offsets, dxts, what):
             raise SystemExit("%s: image %d differs from the expected DXT1 blocks" % (what, i))
 
 
-LAUNCH_MODES = ("graph", "stream", "two_plans")
+LAUNCH_MODES = ("graph", "stream", "two_plans", "two_plan_graphs")
 
 
 def run_batched(gpu, files, dxts, steps, warmup, tune=True, modes=LAUNCH_MODES):
@@ -164,8 +164,9 @@ def run_batched(gpu, files, dxts, steps, warmup, tune=True, modes=LAUNCH_MODES):
     graph (one hipGraph replay a step), stream (the stage kernels launched on the
     stream), two_plans (a second plan bound to the same inputs with its own scratch and
     outputs, on a second stream; step k runs on plan k % 2, so one plan's rANS stage
-    fills the other's assembly tail, as the server's slots run batches).  The step time
-    is the fastest mode.
+    fills the other's assembly tail, as the server's slots run batches), two_plan_graphs
+    (the same with each plan's captured graph replayed on its stream).  The step time is
+    the fastest mode.
     Outputs are poisoned and checked bit-exact after every mode."""
     import gst_amd
     hdrs = [gst_amd.ParseHeader(f) for f in files]
@@ -203,7 +204,7 @@ def run_batched(gpu, files, dxts, steps, warmup, tune=True, modes=LAUNCH_MODES):
         d_out.fill_(0xA5)
         b.decode(gpu.stream)
         check_outputs(gpu, d_out, out_offs, dxts, "batched stream launches")
-    if "two_plans" in modes:
+    if "two_plans" in modes or "two_plan_graphs" in modes:
         s0 = gpu.torch.cuda.current_stream()
         s1 = gpu.torch.cuda.Stream(device=gpu.dev)
         d_out2 = gpu.empty(sum(out_sizes))
@@ -213,19 +214,35 @@ def run_batched(gpu, files, dxts, steps, warmup, tune=True, modes=LAUNCH_MODES):
             b2.autotune(s1.cuda_stream, reps=5)
         plans = ((b, s0), (b2, s1))
 
-        def two_plans(k_steps):
+        def two_plans(k_steps, graphs=False):
             for k in range(k_steps):
                 plan, st = plans[k % 2]
-                plan.decode(st.cuda_stream)
+                if graphs:
+                    plan.launch_graph(st.cuda_stream)
+                else:
+                    plan.decode(st.cuda_stream)
 
-        two_plans(2 * warmup)
-        launch["two_plans"] = gpu.timed_two(s0, s1, lambda: two_plans(steps), steps)
-        d_out.fill_(0xA5)
-        d_out2.fill_(0xA5)
-        two_plans(2)
-        gpu.torch.cuda.synchronize()
-        check_outputs(gpu, d_out, out_offs, dxts, "two plans, first")
-        check_outputs(gpu, d_out2, out_offs, dxts, "two plans, second")
+        def check_both(what):
+            gpu.torch.cuda.synchronize()
+            check_outputs(gpu, d_out, out_offs, dxts, what + ", first")
+            check_outputs(gpu, d_out2, out_offs, dxts, what + ", second")
+
+        if "two_plans" in modes:
+            two_plans(2 * warmup)
+            launch["two_plans"] = gpu.timed_two(s0, s1, lambda: two_plans(steps), steps)
+            d_out.fill_(0xA5)
+            d_out2.fill_(0xA5)
+            two_plans(2)
+            check_both("two plans")
+        if "two_plan_graphs" in modes:
+            for plan, st in plans:
+                plan.capture_graph(st.cuda_stream)
+            two_plans(2 * warmup, graphs=True)
+            launch["two_plan_graphs"] = gpu.timed_two(s0, s1, lambda: two_plans(steps, graphs=True), steps)
+            d_out.fill_(0xA5)
+            d_out2.fill_(0xA5)
+            two_plans(2, graphs=True)
+            check_both("two plans, graph replays")
         b2.close()
         del d_out2
     ms = min(launch.values())
@@ -608,7 +625,8 @@ def main():
             "workload": "C3 batched: %d seeded textures {512x512,1024x1024,512x1024,2048x2048}, "
                         "one pass of the persistent stage kernels per step, %s"
                         % (len(hdrs), {"graph": "hipGraph replays", "stream": "stream launches",
-                                       "two_plans": "two plans in flight on two streams"}
+                                       "two_plans": "two plans in flight on two streams",
+                                       "two_plan_graphs": "two plans in flight on two streams, hipGraph replays"}
                            [min(launch, key=launch.get)]),
             "images": len(hdrs),
             "texels": texels,
