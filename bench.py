@@ -426,12 +426,16 @@ def run_served(gpu, qps_list, duration_s, budget_gib=32, closed_loop=512, tuned=
             kw = dict(plain=(mode == "plain"), copy_to_host=True, **(shape if mode == "batched" else {}))
             res[str(int(qps))] = one("%s @%d req/s" % (mode, qps), kw, dict(qps=qps, seed=int(qps)))
         out[mode] = res
-    # Saturating closed loops: host-submitted inputs (pinned staging -> copy stream ->
-    # HBM, the ingest path), the same with the pool pre-staged in HBM (decode and
-    # scheduling only), and the plain per-request server.
+    # Saturating closed loops: host-submitted inputs (four submitting threads copy them
+    # into pinned staging, the gather kernel reads them into HBM: the ingest path), the
+    # same with a DMA copy per run of adjacent inputs and one submitting thread, the pool
+    # pre-staged in HBM (decode and scheduling only), and the plain per-request server.
     out["closed_loop"] = {
         "depth": closed_loop,
-        "batched_ingest": one("closed loop, ingest", dict(shape), dict(closed_loop=closed_loop, seed=7)),
+        "batched_ingest": one("closed loop, ingest", dict(shape, copy_kernel=True),
+                              dict(closed_loop=closed_loop, seed=7, submit_threads=4)),
+        "batched_ingest_dma": one("closed loop, ingest by DMA copies", dict(shape),
+                                  dict(closed_loop=closed_loop, seed=7)),
         "batched_prestaged": one("closed loop, pre-staged", dict(shape),
                                  dict(closed_loop=closed_loop, seed=7, pool_device=pool_dev)),
         "plain_ingest": one("closed loop, plain", dict(plain=True), dict(closed_loop=64, seed=7)),

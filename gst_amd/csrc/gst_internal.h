@@ -263,6 +263,22 @@ struct CompareArgs {
 };
 hipError_t LaunchCompare(const CompareArgs &a, hipStream_t stream);
 
+// Host-to-device gather in one kernel: job i copies `bytes` from `src` (pinned host
+// memory mapped for the device, 16-byte aligned) to `dst` (device memory, 16-byte
+// aligned).  One launch replaces a DMA copy per job, whose fixed cost dominates copies
+// of a few hundred KiB.
+struct GatherJob {
+  const uint8_t *src;
+  uint8_t *dst;
+  uint64_t bytes;
+};
+constexpr uint32_t kGatherJobs = 64;
+struct GatherArgs {
+  GatherJob job[kGatherJobs];
+  uint32_t n;
+};
+hipError_t LaunchGather(const GatherArgs &a, hipStream_t stream);
+
 // Static device scratch of gst_load_compressed_dxt_noalloc: one array per GPU, held
 // until the process ends.  ResolveNoAllocScratch looks up the array's address on the
 // current device (once per device; the caller makes `device` current);

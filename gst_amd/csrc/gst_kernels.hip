@@ -1645,6 +1645,46 @@ hipError_t LaunchCompare(const CompareArgs &a, hipStream_t stream) {
   return hipGetLastError();
 }
 
+// Work-group row per job, 16-byte words grid-stride with four loads in flight per
+// lane before their stores (reads cross PCIe: latency wants many requests out); the
+// job's last bytes past a whole word go one at a time.
+__global__ __launch_bounds__(256) void GatherKernel(GatherArgs a) {
+  const GatherJob j = a.job[blockIdx.y];
+  const GPtr<const uint4> src = Gbl<const uint4>(j.src);
+  const GPtr<uint4> dst = Gbl<uint4>(j.dst);
+  const uint64_t words = j.bytes / 16;
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  for (; i + 3 * stride < words; i += 4 * stride) {
+    uint4 v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = src[i + k * stride];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) dst[i + k * stride] = v[k];
+  }
+  for (; i < words; i += stride) dst[i] = src[i];
+  if (blockIdx.x == 0 && threadIdx.x < (j.bytes & 15)) {
+    const uint64_t b = words * 16 + threadIdx.x;
+    Gbl<uint8_t>(j.dst)[b] = Gbl<const uint8_t>(j.src)[b];
+  }
+}
+
+hipError_t LaunchGather(const GatherArgs &a, hipStream_t stream) {
+  if (a.n == 0) return hipSuccess;
+  if (a.n > kGatherJobs) return hipErrorInvalidValue;
+  uint64_t most = 0;
+  for (uint32_t i = 0; i < a.n; ++i) {
+    if ((reinterpret_cast<uintptr_t>(a.job[i].src) | reinterpret_cast<uintptr_t>(a.job[i].dst)) & 15)
+      return hipErrorInvalidValue;
+    most = std::max(most, a.job[i].bytes);
+  }
+  // About four words a lane over the largest job, at most 512 work-groups a job.
+  const uint64_t per_group = 256 * 16 * 4;
+  const uint32_t gx = static_cast<uint32_t>(std::min<uint64_t>(512, std::max<uint64_t>(1, (most + per_group - 1) / per_group)));
+  hipLaunchKernelGGL(GatherKernel, dim3(gx, a.n), dim3(256), 0, stream, a);
+  return hipGetLastError();
+}
+
 namespace {
 
 // The no-allocation entry point's scratch.  In host code `scratch` names a host-side

@@ -145,6 +145,10 @@ uint64_t PipelineBytes(const gst_header &h) {
 // A batch that waits on an event re-recorded since by a later group waits for that
 // later copy: longer than needed, never too short.
 constexpr uint32_t kCopyEvents = 256;
+// Inputs waiting for a slot go ahead on the copy stream once this many bytes of them
+// have gathered: a host-to-device copy of a few hundred KiB runs far below the link's
+// rate (fixed cost per copy), one of several MiB near it.
+constexpr uint64_t kCopyGroupBytes = 8ull << 20;
 
 }  // namespace
 
@@ -160,6 +164,8 @@ struct gst_server {
   std::vector<gst_completion> done;
   RangeAllocator dev_alloc, host_alloc;
   uint8_t *d_arena = nullptr, *h_staging = nullptr;
+  uint8_t *h_staging_dev = nullptr;  // the staging's address for device code
+  std::unique_ptr<uint8_t[]> dry_staging;  // a dry run's staging: ordinary memory, same copies
   std::vector<Slot> slots;
   hipStream_t copy_stream = nullptr;
   std::vector<hipEvent_t> copy_ev;  // kCopyEvents
@@ -247,6 +253,8 @@ int IssueCopies(gst_server &s, const std::vector<std::shared_ptr<Req>> &reqs, hi
   if (reqs.empty() || s.cfg.dry_run) return GST_OK;
   size_t i = 0;
   uint64_t bytes = 0, copies = 0;
+  gst::GatherArgs ga;
+  ga.n = 0;
   while (i < reqs.size()) {
     const Req &a = *reqs[i];
     uint64_t h0 = a.host_in_off, d0 = a.dev_in_off, n = a.in_bytes;
@@ -257,11 +265,24 @@ int IssueCopies(gst_server &s, const std::vector<std::shared_ptr<Req>> &reqs, hi
       if (b.host_in_off != h0 + n + gap || b.dev_in_off != d0 + n + gap) break;
       n += gap + b.in_bytes;
     }
-    const hipError_t e = hipMemcpyAsync(s.d_arena + d0, s.h_staging + h0, n, hipMemcpyHostToDevice, stream);
+    hipError_t e = hipSuccess;
+    if (s.cfg.copy_kernel) {
+      ga.job[ga.n++] = {s.h_staging_dev + h0, s.d_arena + d0, n};
+      if (ga.n == gst::kGatherJobs) {
+        e = gst::LaunchGather(ga, stream);
+        ga.n = 0;
+      }
+    } else {
+      e = hipMemcpyAsync(s.d_arena + d0, s.h_staging + h0, n, hipMemcpyHostToDevice, stream);
+    }
     if (e != hipSuccess) return HipErr(e);
     bytes += n;
     ++copies;
     i = j;
+  }
+  if (ga.n) {
+    const hipError_t e = gst::LaunchGather(ga, stream);
+    if (e != hipSuccess) return HipErr(e);
   }
   for (auto &r : reqs) r->copy_pending = false;
   if (event) {
@@ -452,6 +473,8 @@ void SchedulerLoop(gst_server *sp) {
   if (!s.cfg.dry_run) (void)hipSetDevice(0);
   const uint64_t max_blocks = s.cfg.max_batch_blocks;
   auto last_work = Clock::now();
+  std::vector<std::shared_ptr<Req>> backlog;  // admitted, in no batch, input not copied yet
+  uint64_t backlog_bytes = 0;
   for (;;) {
     bool progress = false;
     for (auto &slot : s.slots) progress |= PollSlot(s, slot);
@@ -508,21 +531,34 @@ void SchedulerLoop(gst_server *sp) {
         slot.busy = true;  // reserved until LaunchBatch runs below
       }
     }
-    // Inputs admitted now that no free slot takes in this round go ahead on the copy
-    // stream, overlapping the decodes in flight; the others are copied on their slot's
-    // stream right before their decode (no cross-stream wait on the launch path).
-    fresh.erase(std::remove_if(fresh.begin(), fresh.end(),
-                               [](const std::shared_ptr<Req> &r) { return r->in_batch; }),
-                fresh.end());
-    if (!fresh.empty()) {
-      const int rc = IssueCopies(s, fresh, s.copy_stream, true);
+    // Inputs that no free slot took join the backlog, which goes ahead on the copy stream
+    // (overlapping the decodes in flight) once it holds kCopyGroupBytes; a request a slot
+    // takes first is copied on that slot's stream right before its decode (no
+    // cross-stream wait on the launch path).
+    for (auto &r : fresh) {
+      if (r->in_batch) continue;
+      backlog.push_back(r);
+      backlog_bytes += r->in_bytes;
+    }
+    if (!backlog.empty() && !batches.empty()) {  // drop what a slot took (copied at launch)
+      backlog.erase(std::remove_if(backlog.begin(), backlog.end(),
+                                   [](const std::shared_ptr<Req> &r) { return r->in_batch; }),
+                    backlog.end());
+      backlog_bytes = 0;
+      for (auto &r : backlog) backlog_bytes += r->in_bytes;
+    }
+    if (backlog_bytes >= kCopyGroupBytes) {
+      const int rc = IssueCopies(s, backlog, s.copy_stream, true);
+      std::vector<std::shared_ptr<Req>> copied;
+      copied.swap(backlog);
+      backlog_bytes = 0;
       if (rc != GST_OK) {
         // The copies may be partly enqueued: drain the copy stream, then fail the
         // requests that were to use them.
         (void)hipStreamSynchronize(s.copy_stream);
         std::lock_guard<std::mutex> lk(s.mu);
         const auto now = Clock::now();
-        for (auto &r : fresh) {
+        for (auto &r : copied) {
           auto it = std::find(s.admitted.begin(), s.admitted.end(), r);
           if (it != s.admitted.end()) s.admitted.erase(it);
           r->status = rc;
@@ -619,12 +655,22 @@ int Submit(gst_server *s, const gst_request &q, uint64_t *ticket) {
     s->cv_space.wait(lk);
   }
   if (s->h_staging && r->host_in_bytes) {
+    // The staging range is this request's alone until it is queued: the copy runs
+    // without the lock, so submitters copy in parallel and the scheduler never waits.
+    lk.unlock();
     uint8_t *dst = s->h_staging + r->host_in_off;
     if (s->cfg.plain) {
       // Offsets block content is not read: the decoder derives offsets from the header.
       memcpy(dst + 512, q.file + sizeof(gst_header), q.len - sizeof(gst_header));
     } else {
       memcpy(dst, q.file, q.len);
+    }
+    lk.lock();
+    if (s->stop) {  // shutting down: nothing would complete a request queued now
+      s->host_alloc.Free(r->host_in_off, r->host_in_bytes);
+      if (r->host_out_bytes) s->host_alloc.Free(r->host_out_off, r->host_out_bytes);
+      s->cv_space.notify_all();
+      return GST_ERR_BUSY;
     }
   }
   r->ticket = s->next_ticket++;
@@ -653,12 +699,17 @@ int gst_server_create(const gst_server_config *cfg, gst_server **out) {
   s->dev_alloc.Reset(s->cfg.hbm_budget_bytes);
   s->host_alloc.Reset(s->cfg.host_staging_bytes);
   s->slots.resize(s->cfg.slots);
-  if (!s->cfg.dry_run) {
+  if (s->cfg.dry_run) {
+    s->dry_staging.reset(new uint8_t[s->cfg.host_staging_bytes]);  // pages touched only where copied
+    s->h_staging = s->dry_staging.get();
+  } else {
     int rc = gst_initialize_decoder(0);
     if (rc != GST_OK) return rc;
     const uint32_t max_images = s->cfg.plain ? 1u : s->cfg.max_batch_images;
     hipError_t e = hipMalloc(&s->d_arena, s->cfg.hbm_budget_bytes);
     if (e == hipSuccess) e = hipHostMalloc(&s->h_staging, s->cfg.host_staging_bytes, hipHostMallocDefault);
+    if (e == hipSuccess && s->cfg.copy_kernel)
+      e = hipHostGetDevicePointer(reinterpret_cast<void **>(&s->h_staging_dev), s->h_staging, 0);
     if (e == hipSuccess) e = hipEventCreate(&s->ev_ref);
     if (e == hipSuccess) e = hipEventRecord(s->ev_ref, nullptr);
     if (e == hipSuccess) e = hipEventSynchronize(s->ev_ref);
@@ -794,7 +845,7 @@ int gst_server_destroy(gst_server *s) {
   if (s->copy_stream) (void)hipStreamDestroy(s->copy_stream);
   if (s->ev_ref) (void)hipEventDestroy(s->ev_ref);
   if (s->d_arena) (void)hipFree(s->d_arena);
-  if (s->h_staging) (void)hipHostFree(s->h_staging);
+  if (s->h_staging && !s->dry_staging) (void)hipHostFree(s->h_staging);
   delete s;
   return GST_OK;
 }
@@ -808,22 +859,24 @@ int gst_server_run_load(gst_server *s, const gst_load_spec *spec, double *lat_de
     return GST_ERR_INVALID_ARGUMENT;
   const uint32_t pool_size = spec->pool_size;
   // SplitMix64 draws: file choice and exponential inter-arrival gaps.
-  uint64_t state = spec->seed;
-  auto next_u64 = [&state]() {
+  auto next_u64 = [](uint64_t &state) {
     uint64_t z = (state += 0x9E3779B97F4A7C15ull);
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
     return z ^ (z >> 31);
   };
-  auto uniform = [&]() { return (static_cast<double>(next_u64() >> 11) + 0.5) * (1.0 / 9007199254740992.0); };
+  auto uniform = [&](uint64_t &state) {
+    return (static_cast<double>(next_u64(state) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+  };
+  uint64_t state = spec->seed;
   std::vector<double> cdf(pool_size);
   double acc = 0;
   for (uint32_t i = 0; i < pool_size; ++i) {
     acc += spec->pool_weights ? spec->pool_weights[i] : 1.0;
     cdf[i] = acc;
   }
-  auto pick = [&]() {
-    const double u = uniform() * acc;
+  auto pick = [&](uint64_t &st) {
+    const double u = uniform(st) * acc;
     return static_cast<uint32_t>(std::min<size_t>(std::lower_bound(cdf.begin(), cdf.end(), u) - cdf.begin(),
                                                   pool_size - 1));
   };
@@ -846,8 +899,8 @@ int gst_server_run_load(gst_server *s, const gst_load_spec *spec, double *lat_de
       if (n < buf.size()) return n;
     }
   };
-  auto submit = [&](uint64_t tag) {
-    const uint32_t i = pick();
+  auto submit = [&](uint64_t tag, uint64_t &st) {
+    const uint32_t i = pick(st);
     gst_request q{};
     q.file = spec->pool[i];
     q.len = spec->pool_lens[i];
@@ -869,18 +922,55 @@ int gst_server_run_load(gst_server *s, const gst_load_spec *spec, double *lat_de
         if (next - Clock::now() > std::chrono::microseconds(300))
           std::this_thread::sleep_for(std::chrono::microseconds(100));
       }
-      rc = submit(submitted++);
-      const double gap = -std::log(uniform()) / spec->qps;
+      rc = submit(submitted++, state);
+      const double gap = -std::log(uniform(state)) / spec->qps;
       next += std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double>(gap));
     }
-  } else {
-    for (uint32_t k = 0; k < spec->closed_loop && rc == GST_OK; ++k) rc = submit(submitted++);
+  } else if (spec->submit_threads <= 1) {
+    for (uint32_t k = 0; k < spec->closed_loop && rc == GST_OK; ++k) rc = submit(submitted++, state);
     while (Clock::now() < t_end && rc == GST_OK) {
       const uint64_t before = out.requests;
       collect();
-      for (uint64_t k = before; k < out.requests && rc == GST_OK; ++k) rc = submit(submitted++);
+      for (uint64_t k = before; k < out.requests && rc == GST_OK; ++k) rc = submit(submitted++, state);
       if (out.requests == before) std::this_thread::yield();
     }
+  } else {
+    // This thread collects completions and hands each one back as a submission credit;
+    // the submitting threads (draws seeded seed + thread index) take credits and submit.
+    std::atomic<int64_t> credits{static_cast<int64_t>(spec->closed_loop)};
+    std::atomic<uint64_t> next_tag{0};
+    std::atomic<bool> finish{false};
+    std::atomic<int> first_err{GST_OK};
+    std::vector<std::thread> workers;
+    for (uint32_t t = 0; t < spec->submit_threads; ++t) {
+      workers.emplace_back([&, t]() {
+        uint64_t st = spec->seed + 0x632BE59BD9B4E019ull * (t + 1);
+        while (!finish.load(std::memory_order_acquire)) {
+          int64_t c = credits.load(std::memory_order_relaxed);
+          if (c <= 0 || !credits.compare_exchange_weak(c, c - 1, std::memory_order_acq_rel)) {
+            std::this_thread::yield();
+            continue;
+          }
+          const int e = submit(next_tag.fetch_add(1, std::memory_order_relaxed), st);
+          if (e != GST_OK) {
+            int ok = GST_OK;
+            first_err.compare_exchange_strong(ok, e);
+            finish.store(true, std::memory_order_release);
+          }
+        }
+      });
+    }
+    while (Clock::now() < t_end && !finish.load(std::memory_order_acquire)) {
+      const uint64_t before = out.requests;
+      collect();
+      if (out.requests > before)
+        credits.fetch_add(static_cast<int64_t>(out.requests - before), std::memory_order_acq_rel);
+      else
+        std::this_thread::yield();
+    }
+    finish.store(true, std::memory_order_release);
+    for (auto &w : workers) w.join();
+    rc = first_err.load();
   }
   gst_server_drain(s);
   collect();

@@ -23,6 +23,7 @@ class ServerConfig(ctypes.Structure):
         ("poison_outputs", ctypes.c_uint32),
         ("fail_launch_every", ctypes.c_uint32),
         ("idle_spin_us", ctypes.c_uint32),
+        ("copy_kernel", ctypes.c_uint32),
     ]
 
 
@@ -79,6 +80,7 @@ class LoadSpec(ctypes.Structure):
         ("duration_s", ctypes.c_double),
         ("closed_loop", ctypes.c_uint32),
         ("seed", ctypes.c_uint64),
+        ("submit_threads", ctypes.c_uint32),
     ]
 
 
@@ -118,11 +120,11 @@ class Server:
     def __init__(self, hbm_budget_bytes, host_staging_bytes=0, slots=0, max_batch_images=0,
                  max_batch_blocks=0, copy_to_host=False, auto_release=True, plain=False, dry_run=False,
                  groups_per_wave=0, waves_per_group=0, graph_launch=False, poison_outputs=False,
-                 fail_launch_every=0, idle_spin_us=0):
+                 fail_launch_every=0, idle_spin_us=0, copy_kernel=False):
         cfg = ServerConfig(hbm_budget_bytes, host_staging_bytes, slots, max_batch_images, max_batch_blocks,
                            int(copy_to_host), int(auto_release), int(plain), int(dry_run), groups_per_wave,
                            waves_per_group, int(graph_launch), int(poison_outputs), fail_launch_every,
-                           idle_spin_us)
+                           idle_spin_us, int(copy_kernel))
         self._h = ctypes.c_void_p()
         _check(_lib().gst_server_create(ctypes.byref(cfg), ctypes.byref(self._h)), "gst_server_create")
 
@@ -165,11 +167,12 @@ class Server:
         return st.as_dict()
 
     def run_load(self, pool, weights=None, qps=0.0, duration_s=1.0, closed_loop=0, seed=1, lat_cap=1 << 20,
-                 pool_device=None, pool_expected=None):
+                 pool_device=None, pool_expected=None, submit_threads=0):
         """Poisson (qps) or closed-loop load over `pool`; returns (device latencies,
         host latencies, LoadResult dict).  `pool_device`: the pool files already in device
         memory (pre-staged); `pool_expected`: their expected outputs in device memory, so
-        every completion is compared on the device."""
+        every completion is compared on the device; `submit_threads`: closed loop only,
+        threads that submit in parallel."""
         pool = [bytes(p) for p in pool]
         n = len(pool)
         ptrs = (ctypes.c_char_p * n)(*pool)
@@ -177,7 +180,7 @@ class Server:
         w = (ctypes.c_double * n)(*weights) if weights is not None else None
         dev = (ctypes.c_void_p * n)(*pool_device) if pool_device is not None else None
         exp = (ctypes.c_void_p * n)(*pool_expected) if pool_expected is not None else None
-        spec = LoadSpec(ptrs, lens, w, n, dev, exp, qps, duration_s, closed_loop, seed)
+        spec = LoadSpec(ptrs, lens, w, n, dev, exp, qps, duration_s, closed_loop, seed, submit_threads)
         lat_dev = np.zeros(lat_cap, np.float64)
         lat_host = np.zeros(lat_cap, np.float64)
         res = LoadResult()

@@ -53,6 +53,9 @@ typedef struct gst_server_config {
   uint32_t idle_spin_us;        /* an idle scheduler keeps polling for this long after its last
                                    work before it sleeps on the arrival condition (0: sleeps at
                                    once): trades a host core for the wake-up latency */
+  uint32_t copy_kernel;         /* 1: the inputs admitted together reach the arena through one
+                                   gather kernel reading the pinned staging over PCIe, instead of
+                                   a DMA copy per run of adjacent requests */
 } gst_server_config;
 
 /* One request.  Either `file` holds the whole .gtc on the host (copied to pinned
@@ -131,6 +134,9 @@ typedef struct gst_load_spec {
   double duration_s;
   uint32_t closed_loop;             /* > 0: keep this many requests outstanding instead */
   uint64_t seed;
+  uint32_t submit_threads;          /* closed loop: threads that submit, each copying its requests'
+                                       inputs into the staging in parallel (0 or 1: the calling
+                                       thread alone; one thread's memory copy is about 37 GB/s) */
 } gst_load_spec;
 
 typedef struct gst_load_result {

@@ -204,6 +204,36 @@ void PrestagedAndLoad() {
   CHECK(gst_server_destroy(s) == GST_OK);
 }
 
+// The load generator's closed loop with several submitting threads copying inputs into
+// the staging in parallel (a dry run's staging is ordinary memory, so the copies run).
+void LoadWithSubmitThreads() {
+  const std::vector<uint8_t> a = FakeFile(512, 512), b = FakeFile(1024, 1024);
+  gst_server_config cfg{};
+  cfg.hbm_budget_bytes = 256ull << 20;
+  cfg.host_staging_bytes = 8ull << 20;  // small: submitters also wait for space
+  cfg.dry_run = 1;
+  cfg.auto_release = 1;
+  gst_server *s = nullptr;
+  CHECK(gst_server_create(&cfg, &s) == GST_OK);
+  const uint8_t *pool[2] = {a.data(), b.data()};
+  const size_t lens[2] = {a.size(), b.size()};
+  gst_load_spec spec{};
+  spec.pool = pool;
+  spec.pool_lens = lens;
+  spec.pool_size = 2;
+  spec.duration_s = 0.2;
+  spec.closed_loop = 64;
+  spec.seed = 5;
+  spec.submit_threads = 4;
+  gst_load_result res{};
+  CHECK(gst_server_run_load(s, &spec, nullptr, nullptr, 0, &res) == GST_OK);
+  gst_server_stats st{};
+  CHECK(gst_server_stats_get(s, &st) == GST_OK);
+  CHECK(res.requests >= 64 && res.failed == 0);
+  CHECK(st.submitted == st.completed && st.completed == res.requests);
+  CHECK(gst_server_destroy(s) == GST_OK);
+}
+
 }  // namespace
 
 int main() {
@@ -217,6 +247,8 @@ int main() {
   DestroyWithWaitingRequests();
   fprintf(stderr, "pre-staged requests and the load generator\n");
   PrestagedAndLoad();
+  fprintf(stderr, "load generator with several submitting threads\n");
+  LoadWithSubmitThreads();
   if (g_failures.load()) {
     fprintf(stderr, "%d checks failed\n", g_failures.load());
     return 1;

@@ -103,6 +103,12 @@ def test_poisson_load_generator_dry_run():
     with Server(1 << 30, dry_run=True) as s:
         _, _, res = s.run_load(files, closed_loop=8, duration_s=0.1, seed=3)
         assert res["requests"] >= 8
+    # Several submitting threads: every submission completes once, none is lost.
+    with Server(1 << 30, dry_run=True) as s:
+        _, _, res = s.run_load(files, closed_loop=64, duration_s=0.2, seed=4, submit_threads=4)
+        st = s.stats()
+        assert res["requests"] >= 64 and res["failed"] == 0
+        assert st["submitted"] == st["completed"] == res["requests"]
 
 
 def test_failed_launch_fails_its_batch_and_frees_its_memory():
@@ -169,16 +175,18 @@ def served_corpus():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("graph_launch", [False, True])
-def test_served_results_equal_alone_and_in_the_mix(device, served_corpus, graph_launch):
+@pytest.mark.parametrize("graph_launch,copy_kernel", [(False, False), (True, False), (False, True)])
+def test_served_results_equal_alone_and_in_the_mix(device, served_corpus, graph_launch, copy_kernel):
     """Each request decoded alone (one at a time) and in a concurrent stream of
-    arrivals gives the same bit-exact blocks, with graph replays or stream launches.
-    Outputs are poisoned before every decode, and the in-mix run also compares each
-    output with its expected blocks on the device."""
+    arrivals gives the same bit-exact blocks, with graph replays or stream launches,
+    inputs copied in by DMA or by the gather kernel.  Outputs are poisoned before every
+    decode, and the in-mix run also compares each output with its expected blocks on
+    the device."""
     files, dxts = served_corpus
     download = _hip_download()
     alone = {}
-    with Server(1 << 30, auto_release=False, graph_launch=graph_launch, poison_outputs=True) as s:
+    with Server(1 << 30, auto_release=False, graph_launch=graph_launch, poison_outputs=True,
+                copy_kernel=copy_kernel) as s:
         for i, f in enumerate(files):
             s.submit(f, tag=i)
             s.drain()
@@ -189,7 +197,8 @@ def test_served_results_equal_alone_and_in_the_mix(device, served_corpus, graph_
     assert [alone[i] for i in range(len(files))] == dxts
 
     d_expected = [device.upload(d) for d in dxts]
-    with Server(1 << 30, auto_release=False, slots=3, graph_launch=graph_launch, poison_outputs=True) as s:
+    with Server(1 << 30, auto_release=False, slots=3, graph_launch=graph_launch, poison_outputs=True,
+                copy_kernel=copy_kernel) as s:
         order = list(range(len(files))) * 4
         np.random.default_rng(0).shuffle(order)
         for i in order:
@@ -250,8 +259,9 @@ def test_prestaged_inputs_decode_without_copies(device, served_corpus):
 
 @pytest.mark.gpu
 def test_served_poisson_load_checks_every_completion(device, served_corpus):
-    """The load generator under Poisson arrivals and in a closed loop, host-submitted
-    and pre-staged: every completion is compared with its expected blocks on the device."""
+    """The load generator under Poisson arrivals and in closed loops, host-submitted
+    (inputs copied by DMA or by the gather kernel) and pre-staged: every completion is
+    compared with its expected blocks on the device."""
     files, dxts = served_corpus
     d_files = [device.upload(f) for f in files]
     d_expected = [device.upload(d) for d in dxts]
@@ -260,6 +270,11 @@ def test_served_poisson_load_checks_every_completion(device, served_corpus):
         _, _, res = s.run_load(files, qps=2000.0, duration_s=0.3, seed=11, pool_expected=exp)
         assert res["requests"] > 100 and res["failed"] == 0
         assert res["verified"] == res["requests"] and res["mismatches"] == 0
+    with Server(1 << 30, poison_outputs=True, copy_kernel=True) as s:
+        _, _, res = s.run_load(files, closed_loop=64, duration_s=0.3, seed=13, pool_expected=exp)
+        assert res["requests"] > 64 and res["failed"] == 0
+        assert res["verified"] == res["requests"] and res["mismatches"] == 0
+        assert s.stats()["h2d_copies"] > 0
     with Server(1 << 30, poison_outputs=True) as s:
         _, _, res = s.run_load(files, closed_loop=32, duration_s=0.3, seed=12, pool_expected=exp,
                                pool_device=[t.data_ptr() for t in d_files])
@@ -268,12 +283,13 @@ def test_served_poisson_load_checks_every_completion(device, served_corpus):
 
 
 @pytest.mark.gpu
-def test_plain_mode_matches(device, served_corpus):
+@pytest.mark.parametrize("copy_kernel", [False, True])
+def test_plain_mode_matches(device, served_corpus, copy_kernel):
     files, dxts = served_corpus
     g = load_golden("ref_wide")
     download = _hip_download()
     d_expected = device.upload(dxts[-2])
-    with Server(1 << 28, plain=True, auto_release=False, poison_outputs=True) as s:
+    with Server(1 << 28, plain=True, auto_release=False, poison_outputs=True, copy_kernel=copy_kernel) as s:
         s.submit(g["gtc"].tobytes(), tag=0)
         s.submit_request(files[-2], tag=1, d_expected=d_expected.data_ptr())  # 4096x4096
         s.drain()
