@@ -246,15 +246,13 @@ def run_batched(gpu, files, dxts, steps, warmup, tune=True, modes=LAUNCH_MODES):
         b2.close()
         del d_out2
     ms = min(launch.values())
-    # Per-stage kernel time in the pipeline's order and cache state: the five launches
-    # with a hipEvent recorded on the decode stream before and after each, mean of 20.
-    reps = 20
-    stage = {k: None for k in gst_amd.STAGES}
+    # Per-stage kernel time in the pipeline's order and cache state: 200 decodes back to
+    # back with a timing-only hipEvent (no cache write-back) recorded on the decode stream
+    # after every launched stage, after `warmup` untimed ones; mean per stage.
     d_out.fill_(0xA5)
-    for _ in range(reps):
-        for k, v in b.decode_timed(gpu.stream).items():
-            if v is not None:
-                stage[k] = (stage[k] or 0.0) + v / reps
+    for _ in range(warmup):
+        b.decode(gpu.stream)
+    stage = b.decode_timed(gpu.stream, reps=200)
     check_outputs(gpu, d_out, out_offs, dxts, "batched timed")
     dev_bytes = b.device_bytes
     b.close()
@@ -544,8 +542,11 @@ def run_ans_api(files, reps=3):
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=500, help="timed steps per launch mode")
+    ap.add_argument("--warmup", type=int, default=300,
+                    help="untimed steps before each launch mode's timed steps: the GPU's clocks take "
+                         "about 25 ms of back-to-back decodes to settle after the autotuner's "
+                         "stop-and-go, and short timed windows inside that ramp read up to 13%% slow")
     ap.add_argument("--images", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the per-image comparison")

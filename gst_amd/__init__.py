@@ -152,6 +152,7 @@ def _declare(lib):
         "gst_batch_capture_graph": ([_P, _P], ctypes.c_int),
         "gst_batch_launch_graph": ([_P, _P], ctypes.c_int),
         "gst_batch_decode_timed": ([_P, _P, ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
+        "gst_batch_decode_timed_reps": ([_P, _P, _U32, ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
         "gst_batch_error_bits": ([_P, _P, ctypes.POINTER(_U32)], ctypes.c_int),
         "gst_batch_autotune": ([_P, _P, _U32, ctypes.POINTER(TuneResult)], ctypes.c_int),
         "gst_batch_device_bytes": ([_P], _SZ),
@@ -372,9 +373,13 @@ class Batch:
     def launch_graph(self, stream):
         _check(lib().gst_batch_launch_graph(self._h, stream), "gst_batch_launch_graph")
 
-    def decode_timed(self, stream):
+    def decode_timed(self, stream, reps=1):
+        """Per-stage ms of one decode, or the mean over `reps` decodes run back to back."""
         ms = (ctypes.c_float * 5)()
-        _check(lib().gst_batch_decode_timed(self._h, stream, ms), "gst_batch_decode_timed")
+        if reps == 1:
+            _check(lib().gst_batch_decode_timed(self._h, stream, ms), "gst_batch_decode_timed")
+        else:
+            _check(lib().gst_batch_decode_timed_reps(self._h, stream, reps, ms), "gst_batch_decode_timed_reps")
         return _stage_times(ms)
 
     def error_bits(self, stream):

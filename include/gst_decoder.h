@@ -237,6 +237,10 @@ int gst_batch_launch_graph(gst_batch *b, gst_stream stream);
  * launches nothing in the plan's shape (tables built inside the rANS kernel, an index
  * scan or wavelet folded into the assembly) reports NaN.  Blocking. */
 int gst_batch_decode_timed(gst_batch *b, gst_stream stream, float ms[5]);
+/* The same over `reps` (1..4096) decodes launched back to back, one host wait at the
+ * end: mean per stage.  The events only time (no system-scope fence), so the kernels
+ * run as they do without them, apart from the events' own packets. */
+int gst_batch_decode_timed_reps(gst_batch *b, gst_stream stream, uint32_t reps, float ms[5]);
 int gst_batch_error_bits(gst_batch *b, gst_stream stream, uint32_t *bits);
 /* Non-blocking form: enqueues on `stream` a copy of the error bits into `host_bits`
  * (pinned host memory) followed by their reset; read it once `stream` has passed. */

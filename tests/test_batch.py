@@ -120,6 +120,14 @@ def test_decode_timed_reports_launched_stages_only(device, opts, launched):
         assert all(ms[k] > 0 for k in launched)
         assert [device.download(o) for o in outs] == dxts
         assert b.error_bits(device.stream) == 0
+        # Several decodes back to back: the same stages, mean times, the same blocks.
+        for o in outs:
+            device.poison(o)
+        ms = b.decode_timed(device.stream, reps=5)
+        assert {k for k, v in ms.items() if v is not None} == launched
+        assert all(ms[k] > 0 for k in launched)
+        assert [device.download(o) for o in outs] == dxts
+        assert b.error_bits(device.stream) == 0
     finally:
         b.close()
 
