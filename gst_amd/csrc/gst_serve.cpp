@@ -175,6 +175,7 @@ struct gst_server {
   std::atomic<bool> stop{false};
   uint64_t next_ticket = 1;
   uint64_t in_flight = 0;  // submitted, not completed
+  uint32_t staging_copies = 0;  // submits copying into the staging without the lock
   hipEvent_t ev_ref = nullptr;
   std::vector<std::pair<double, double>> busy;  // GPU-timeline intervals (ms)
   gst_server_stats st{};
@@ -657,6 +658,8 @@ int Submit(gst_server *s, const gst_request &q, uint64_t *ticket) {
   if (s->h_staging && r->host_in_bytes) {
     // The staging range is this request's alone until it is queued: the copy runs
     // without the lock, so submitters copy in parallel and the scheduler never waits.
+    // Destroy waits for such copies before it frees the staging.
+    s->staging_copies++;
     lk.unlock();
     uint8_t *dst = s->h_staging + r->host_in_off;
     if (s->cfg.plain) {
@@ -666,6 +669,7 @@ int Submit(gst_server *s, const gst_request &q, uint64_t *ticket) {
       memcpy(dst, q.file, q.len);
     }
     lk.lock();
+    s->staging_copies--;
     if (s->stop) {  // shutting down: nothing would complete a request queued now
       s->host_alloc.Free(r->host_in_off, r->host_in_bytes);
       if (r->host_out_bytes) s->host_alloc.Free(r->host_out_off, r->host_out_bytes);
@@ -821,8 +825,10 @@ int gst_server_destroy(gst_server *s) {
   if (!s) return GST_ERR_INVALID_ARGUMENT;
   if (s->sched.joinable()) {
     {
-      std::lock_guard<std::mutex> lk(s->mu);
+      std::unique_lock<std::mutex> lk(s->mu);
       s->stop = true;
+      s->cv_space.notify_all();
+      s->cv_space.wait(lk, [s] { return s->staging_copies == 0; });  // no submit still copying
     }
     s->cv_work.notify_all();
     s->cv_space.notify_all();

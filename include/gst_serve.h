@@ -107,7 +107,9 @@ typedef struct gst_server_stats {
 int gst_server_create(const gst_server_config *cfg, gst_server **out);
 /* Copies `file` into pinned staging and queues it; returns at once.  Blocks only
  * while host staging is full.  GST_ERR_BAD_HEADER for an invalid file,
- * GST_ERR_OUT_OF_MEMORY if the request can never fit the budget or a batch. */
+ * GST_ERR_OUT_OF_MEMORY if the request can never fit the budget or a batch.  Any number
+ * of threads may submit at once (the copies run in parallel); a submit still running
+ * when gst_server_destroy starts returns GST_ERR_BUSY, and none may start after. */
 int gst_server_submit(gst_server *s, const uint8_t *file, size_t len, uint64_t tag, uint64_t *ticket);
 /* General form: a host or pre-staged device input, with an optional expected output.
  * GST_ERR_INVALID_ARGUMENT for a misaligned d_expected. */
