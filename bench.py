@@ -445,6 +445,43 @@ def run_served(gpu, qps_list, duration_s, budget_gib=32, closed_loop=512, tuned=
     return out
 
 
+def run_h2d_rates(gpu, total=1 << 30):
+    """GB/s of uploading `total` bytes of pinned host memory to the device in ranges of one
+    size: one DMA copy per range (torch's copy of a pinned tensor slice) against the gather
+    kernel (gst_upload_gather, 64 ranges a launch).  Best of 3 passes each; the last
+    pass's bytes are checked."""
+    import gst_amd
+    torch = gpu.torch
+    host = torch.empty(total, dtype=torch.uint8).pin_memory()
+    host[::4096] = torch.arange(total // 4096, dtype=torch.int64).to(torch.uint8)
+    dev = gpu.empty(total)
+    out = {}
+    for chunk in (64 << 10, 256 << 10, 1 << 20, 4 << 20, 16 << 20):
+        n = total // chunk
+        hp, dp = host.data_ptr(), dev.data_ptr()
+
+        def dma():
+            for i in range(n):
+                dev[i * chunk:(i + 1) * chunk].copy_(host[i * chunk:(i + 1) * chunk], non_blocking=True)
+
+        def gather():
+            gst_amd.UploadGather([hp + i * chunk for i in range(n)], [dp + i * chunk for i in range(n)],
+                                 [chunk] * n, gpu.stream)
+
+        row = {}
+        for name, fn in (("dma", dma), ("gather", gather)):
+            ms = min(gpu.timed(fn, 1) for _ in range(3))
+            dev.fill_(0xA5)
+            fn()
+            torch.cuda.synchronize()
+            if not torch.equal(dev[::4096].cpu(), host[::4096]):
+                raise SystemExit("h2d %s: bytes differ at chunk %d" % (name, chunk))
+            row[name] = round(total / (ms * 1e-3) / 1e9, 1)
+        out["%d KiB" % (chunk >> 10)] = row
+        log("h2d %d KiB: %s" % (chunk >> 10, row))
+    return out
+
+
 def run_stage_kernels(gpu, reps=20):
     """The reference's five stage launches one at a time (codec/decoder.cpp:165-417,
     gst_stage_* here) on the single-image configurations: C1 (the 512x512 seed-0
@@ -559,6 +596,9 @@ def main():
     ap.add_argument("--stage-kernels", action="store_true",
                     help="only time the five per-stage kernels (gst_stage_*) on the 512x512 and 4096x4096 "
                          "single-image configurations and print their GB/s")
+    ap.add_argument("--h2d-rates", action="store_true",
+                    help="only time host-to-device uploads of 1 GiB of pinned memory in ranges of one "
+                         "size (64 KiB .. 16 MiB): a DMA copy per range against gst_upload_gather")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device-to-device copy of 512 MiB (5x): a kernel of known traffic "
                          "against which HBM byte counters are checked")
@@ -582,6 +622,9 @@ def main():
     from gst_amd import encode
     if not gst_amd.InitializeDecoder(0):
         raise SystemExit("cuda:0 is not a usable gfx950")
+    if args.h2d_rates:
+        print(json.dumps({"h2d_GBs": run_h2d_rates(gpu)}), flush=True)
+        return
     if args.stage_kernels:
         print(json.dumps({"stage_kernels": run_stage_kernels(gpu)}), flush=True)
         return

@@ -140,6 +140,8 @@ def _declare(lib):
         "gst_staging_size": ([_HP, _U32, ctypes.POINTER(_SZ), ctypes.POINTER(_SZ)], ctypes.c_int),
         "gst_pack_staging": ([ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(_SZ), _U32, _P, _SZ],
                              ctypes.c_int),
+        "gst_upload_gather": ([ctypes.POINTER(_P), ctypes.POINTER(_P), ctypes.POINTER(ctypes.c_uint64), _U32, _P],
+                              ctypes.c_int),
         "gst_stage_build_table": ([_P, _P, _P, _P], ctypes.c_int),
         "gst_stage_ans_decode": ([_HP, _P, _P, _P, _P, _P, _P], ctypes.c_int),
         "gst_stage_decode_indices": ([_HP, _P, _P, _P, _P], ctypes.c_int),
@@ -293,6 +295,15 @@ def StagingSize(hdrs):
     total, off = _SZ(), _SZ()
     _check(lib().gst_staging_size(arr, len(hdrs), ctypes.byref(total), ctypes.byref(off)), "StagingSize")
     return total.value, off.value
+
+
+def UploadGather(host_ptrs, dev_ptrs, sizes, stream):
+    """gst_upload_gather: pinned host ranges -> device ranges in one gather kernel per 64."""
+    n = len(sizes)
+    src = (_P * max(n, 1))(*host_ptrs)
+    dst = (_P * max(n, 1))(*dev_ptrs)
+    nb = (ctypes.c_uint64 * max(n, 1))(*[int(b) for b in sizes])
+    _check(lib().gst_upload_gather(src, dst, nb, n, stream), "gst_upload_gather")
 
 
 def PackStaging(files):

@@ -529,6 +529,43 @@ int gst_staging_size(const gst_header *hdrs, uint32_t n, size_t *total, size_t *
   return GST_OK;
 }
 
+int gst_upload_gather(const void *const *host_src, void *const *dev_dst, const uint64_t *bytes, uint32_t n,
+                      gst_stream stream) {
+  if (n && (!host_src || !dev_dst || !bytes)) return GST_ERR_INVALID_ARGUMENT;
+  if (n == 0) return GST_OK;
+  const int rc = EnsureInit();
+  if (rc != GST_OK) return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  gst::GatherArgs ga;
+  ga.n = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    if (bytes[i] == 0) continue;
+    if (!host_src[i] || !dev_dst[i] ||
+        ((reinterpret_cast<uintptr_t>(host_src[i]) | reinterpret_cast<uintptr_t>(dev_dst[i])) & 15))
+      return GST_ERR_INVALID_ARGUMENT;
+    // Only host memory the runtime pinned has a device address; a kernel must never read
+    // anything else (pageable memory would fault the device).
+    hipPointerAttribute_t attr{};
+    void *src = nullptr;
+    hipDeviceptr_t base = nullptr;
+    size_t size = 0;
+    if (hipPointerGetAttributes(&attr, host_src[i]) != hipSuccess || attr.type != hipMemoryTypeHost ||
+        hipHostGetDevicePointer(&src, const_cast<void *>(host_src[i]), 0) != hipSuccess || !src ||
+        hipMemGetAddressRange(&base, &size, src) != hipSuccess ||
+        static_cast<uint8_t *>(src) + bytes[i] > static_cast<uint8_t *>(base) + size) {
+      (void)hipGetLastError();
+      return GST_ERR_INVALID_ARGUMENT;  // not pinned, or the range runs past its allocation
+    }
+    ga.job[ga.n++] = {static_cast<const uint8_t *>(src), static_cast<uint8_t *>(dev_dst[i]), bytes[i]};
+    if (ga.n == gst::kGatherJobs) {
+      GST_HIP(gst::LaunchGather(ga, s));
+      ga.n = 0;
+    }
+  }
+  if (ga.n) GST_HIP(gst::LaunchGather(ga, s));
+  return GST_OK;
+}
+
 int gst_pack_staging(const uint8_t *const *files, const size_t *lens, uint32_t n, uint8_t *out,
                      size_t cap) {
   if (!files || !lens || n == 0 || !out) return GST_ERR_INVALID_ARGUMENT;

@@ -146,6 +146,17 @@ int gst_staging_size(const gst_header *hdrs, uint32_t n, size_t *total, size_t *
 int gst_pack_staging(const uint8_t *const *files, const size_t *lens, uint32_t n,
                      uint8_t *out, size_t cap);
 
+/* Upload of n ranges of pinned host memory (hipHostMalloc, 16-byte aligned) into device
+ * memory (16-byte aligned) by a gather kernel on `stream`, one launch per 64 ranges: the
+ * device reads the host memory over PCIe, about 50 GB/s whatever the range sizes, where a
+ * DMA copy per range has a fixed cost of about 10 us (6 GB/s at 64 KiB).  The reference
+ * uploads its pinned pages with one buffer copy each (demo/photos_sf.cpp:707-722).
+ * Ranges of 0 bytes are skipped; GST_ERR_INVALID_ARGUMENT for a misaligned pointer or a
+ * source the device cannot address.  Asynchronous: the host memory must stay unchanged
+ * until `stream` has passed the copy. */
+int gst_upload_gather(const void *const *host_src, void *const *dev_dst, const uint64_t *bytes, uint32_t n,
+                      gst_stream stream);
+
 /* ---- one launch per stage (per image; for parity tests and custom pipelines) ---
  * d_streams points at the image's stream section (Y stream first).  Scratch layout:
  * d_tables 4 x 2048 u32 plus 4 u32 status words; d_symbols 7*nv+P bytes in the
