@@ -356,6 +356,7 @@ __device__ __forceinline__ void DecodeHalfGroups(const uint32_t *tab, uint16_t *
   uint16_t *const ringA = ring_wave;
   uint16_t *const ringB = ring_wave + kRingWords;
   uint32_t topA = ring.topA, topB = ring.topB;
+  const uint32_t startA = pA, startB = pB;  // a pointer above its start has wrapped below 0
   // Chunks top-2 and top-3 go in flight in two fixed registers by chunk parity (even
   // chunks in pXE, odd in pXO): a load lands in the register its chunk is later
   // stored from, so no register moves sit between in-flight loads.  The wait before a
@@ -490,7 +491,7 @@ __device__ __forceinline__ void DecodeHalfGroups(const uint32_t *tab, uint16_t *
         dst[i] = make_uint4(acc[4 * i], acc[4 * i + 1], acc[4 * i + 2], acc[4 * i + 3]);
     }
   }
-  const bool under = lane < 32 ? (pA < loA) : (pB < loB);
+  const bool under = lane < 32 ? (pA < loA || pA > startA) : (pB < loB || pB > startB);
   if (active && under && j == 0) atomicOr(err, kErrStreamOffsets);
   if (kSum) {
     // Inclusive prefix of the lane totals over the half-wave's 32 lanes.

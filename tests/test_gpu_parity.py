@@ -265,6 +265,22 @@ def test_bad_group_offsets_are_reported(device):
         assert e.value.code == -3
 
 
+@pytest.mark.parametrize("end", [128, 512])
+def test_group_pointer_running_below_the_stream_start_is_reported(device, end):
+    """Y group 0 cut to `end` bytes (its words really span 2328): its lanes renormalise
+    past the start of the stream, so the word pointer runs below 0.  The oracle reports
+    the file as corrupt; so must the device, although the pointer wraps round."""
+    gtc = load_golden("ref_c0")["gtc"].tobytes()
+    y0 = 28 + 2048
+    assert struct.unpack_from("<I", gtc, y0)[0] == 2328
+    bad = _corrupt(gtc, lambda b: struct.pack_into("<I", b, y0, end))
+    rc, _, _ = go.decode_dxt(bad, nthreads=4)
+    assert rc == go.ERR_STREAM
+    with pytest.raises(gst_amd.GstError) as e:
+        gst_amd.DecompressDXT(bad)
+    assert e.value.code == -3
+
+
 @pytest.mark.parametrize("stream", ["palette", "indices"])
 def test_garbled_renormalisation_words_stay_in_bounds(device, stream):
     """Noise in a stream's words decodes to other symbols; palette indices that leave
