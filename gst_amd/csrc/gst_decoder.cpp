@@ -854,7 +854,9 @@ int gst_batch_create(const gst_batch_config *cfg, gst_batch **out) {
   b->grids[gst::kStageAns] = cfg->grid_ans ? cfg->grid_ans : cus * 8;
   b->grids[gst::kStageIndices] = cfg->grid_indices ? cfg->grid_indices : cus * 4;
   b->grids[gst::kStageWavelet] = cfg->grid_wavelet ? cfg->grid_wavelet : cus * 16;
-  b->grids[gst::kStageAssemble] = cfg->grid_assemble ? cfg->grid_assemble : cus * 8;
+  // 16 per CU: past the 6 fused work-groups a CU holds at once, so the hardware keeps
+  // every CU fed while units remain (a persistent 8 per CU left a round-trip tail).
+  b->grids[gst::kStageAssemble] = cfg->grid_assemble ? cfg->grid_assemble : cus * 16;
   const uint64_t cap = cfg->capacity_images, B = cfg->capacity_blocks;
   const uint64_t Pcap = cfg->capacity_palette ? cfg->capacity_palette : 4 * B;
   size_t off = 0;

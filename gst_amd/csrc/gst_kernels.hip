@@ -523,6 +523,24 @@ __device__ __forceinline__ uint32_t XcdContiguous(uint32_t b, uint32_t n) {
   return (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + idx;
 }
 
+// The same split by the stage's actual unit count, for grids of any size (persistent
+// grids below it, plan grids above it): XCD x owns the x-th of 8 contiguous ranges of
+// [0, units) and its work-groups stride through that range, so every XCD gets an equal
+// share whatever the grid (a split by the grid's size would leave whole XCDs idle when
+// the grid exceeds the units, and one XCD with the last round when it falls short).
+// Grids under 8 work-groups leave XCDs without work-groups: they stride over all units.
+struct XcdSpan {
+  uint32_t first, end, stride;
+};
+__device__ __forceinline__ XcdSpan XcdBalanced(uint32_t b, uint32_t grid, uint32_t units) {
+  if (grid < kXcds) return {b, units, grid};
+  const uint32_t q = units / kXcds, r = units % kXcds, xcd = b % kXcds, idx = b / kXcds;
+  const uint32_t lo = xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q;
+  const uint32_t n = q + (xcd < r ? 1u : 0u);
+  const uint32_t groups_here = (grid - xcd + kXcds - 1) / kXcds;  // work-groups with b % 8 == xcd
+  return {lo + idx, lo + n, groups_here};
+}
+
 __device__ __forceinline__ const uint8_t *Uniform(const uint8_t *p) {
   const uint64_t v = reinterpret_cast<uint64_t>(p);
   const uint64_t lo = Uniform(static_cast<uint32_t>(v));
@@ -1342,7 +1360,11 @@ __global__ __launch_bounds__(kTileFusedWaves * 64, 5) void WaveletAssembleKernel
   const uint32_t lane = threadIdx.x & 63, half = lane >> 5, r = lane & 31;
   const uint32_t wave = Uniform(threadIdx.x >> 6);
   uint32_t *const L = lds[wave][half];
-  for (uint32_t u = XcdContiguous(blockIdx.x, gridDim.x);; u += gridDim.x) {
+  // Work units are pairs of assembly units (every image has an even number of them).
+  const uint32_t pairs = (a.plan ? Uniform(Gbl<const uint32_t>(&a.plan->units[kStageAssemble])[0])
+                                 : StageUnitsOf(a.single, kStageAssemble, 1)) / 2;
+  const XcdSpan span = XcdBalanced(blockIdx.x, gridDim.x, pairs);
+  for (uint32_t u = span.first; u < span.end; u += span.stride) {
     const Located loc = Locate(a, kStageAssemble, 2 * u, 1);
     if (!loc.valid) break;
     const DevDesc d = LoadDesc(a, loc.img);
