@@ -711,7 +711,10 @@ int gst_server_create(const gst_server_config *cfg, gst_server **out) {
     if (rc != GST_OK) return rc;
     const uint32_t max_images = s->cfg.plain ? 1u : s->cfg.max_batch_images;
     hipError_t e = hipMalloc(&s->d_arena, s->cfg.hbm_budget_bytes);
-    if (e == hipSuccess) e = hipHostMalloc(&s->h_staging, s->cfg.host_staging_bytes, hipHostMallocDefault);
+    // The gather kernel reads the staging directly: coherent memory is never held in a
+    // GPU cache, so a staging range refilled by a later submit is always read afresh.
+    const unsigned staging_flags = s->cfg.copy_kernel ? hipHostMallocCoherent : hipHostMallocDefault;
+    if (e == hipSuccess) e = hipHostMalloc(&s->h_staging, s->cfg.host_staging_bytes, staging_flags);
     if (e == hipSuccess && s->cfg.copy_kernel)
       e = hipHostGetDevicePointer(reinterpret_cast<void **>(&s->h_staging_dev), s->h_staging, 0);
     if (e == hipSuccess) e = hipEventCreate(&s->ev_ref);
