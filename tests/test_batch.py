@@ -237,10 +237,11 @@ def test_full_plan_of_many_small_images(device):
         b.close()
 
 
-@pytest.mark.parametrize("ans_grid,asm_grid", [(9, 13), (3, 1), (1568, 7)])
+@pytest.mark.parametrize("ans_grid,asm_grid", [(9, 13), (3, 1), (1568, 7), (64, 5003), (64, 8)])
 def test_odd_persistent_grids(device, ans_grid, asm_grid):
-    """Grids that are not multiples of the 8 XCDs: the fused kernel's XCD-contiguous unit
-    order must still visit every unit exactly once, pass after pass."""
+    """Grids that are not multiples of the 8 XCDs, below 8, exactly 8, and far above the
+    unit count: the fused kernel's per-XCD unit ranges must still visit every unit exactly
+    once, pass after pass."""
     files, dxts = encode.make_textures(MIXED[:5])
     blocks, pal = _capacity(files)
     b = gst_amd.Batch(len(files), blocks, pal, grids={"ans_decode": ans_grid, "assemble": asm_grid})
