@@ -696,7 +696,10 @@ int gst_server_create(const gst_server_config *cfg, gst_server **out) {
   if (!cfg || !out || cfg->hbm_budget_bytes == 0) return GST_ERR_INVALID_ARGUMENT;
   std::unique_ptr<gst_server> s(new gst_server());
   s->cfg = *cfg;
-  if (!s->cfg.slots) s->cfg.slots = s->cfg.plain ? 1 : 3;
+  // Four batches in flight keep the device busier than three under a saturating load
+  // (size mix, pre-staged, 512 outstanding, one box: 1.32 against 1.23 M Mtexel/s; six no
+  // better, eight slower).
+  if (!s->cfg.slots) s->cfg.slots = s->cfg.plain ? 1 : 4;
   if (!s->cfg.max_batch_images) s->cfg.max_batch_images = 256;
   if (!s->cfg.max_batch_blocks) s->cfg.max_batch_blocks = 16ull << 20;
   if (!s->cfg.host_staging_bytes) s->cfg.host_staging_bytes = 1ull << 30;

@@ -34,7 +34,7 @@ typedef struct gst_server gst_server;
 typedef struct gst_server_config {
   uint64_t hbm_budget_bytes;    /* device arena for request inputs + outputs */
   uint64_t host_staging_bytes;  /* pinned host staging for submitted inputs (0: 1 GiB) */
-  uint32_t slots;               /* concurrent batches in flight (0: 3) */
+  uint32_t slots;               /* concurrent batches in flight (0: 4; plain mode 1) */
   uint32_t max_batch_images;    /* per launch (0: 256) */
   uint64_t max_batch_blocks;    /* per launch, sum of w*h/16 (0: 16 Mi) */
   uint32_t copy_to_host;        /* 1: copy each output to pinned host memory after decode */
