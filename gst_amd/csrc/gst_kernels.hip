@@ -1703,9 +1703,10 @@ hipError_t LaunchGather(const GatherArgs &a, hipStream_t stream) {
   }
   // A few work-groups keep the link busy (each has 16 KiB of loads in flight; about
   // 2 us x 55 GB/s is needed), so the gather takes at most 128 work-groups in all and
-  // leaves the CUs to the decodes it runs beside: at most 128 / n a job, and no more than
-  // one per 64 KiB of the largest job.
-  const uint64_t per_group = 64 << 10;
+  // leaves the CUs to the decodes it runs beside: at most 128 / n a job.  A small job gets
+  // one work-group per 4 KiB (a word a lane), so all its loads go out at once: one link
+  // round trip, shorter than a DMA copy's fixed cost.
+  const uint64_t per_group = 4 << 10;
   const uint32_t gx = static_cast<uint32_t>(
       std::max<uint64_t>(1, std::min<uint64_t>(128 / a.n, (most + per_group - 1) / per_group)));
   hipLaunchKernelGGL(GatherKernel, dim3(gx, a.n), dim3(256), 0, stream, a);
