@@ -286,6 +286,33 @@ def test_two_plans_in_flight(device):
             p.close()
 
 
+def test_full_size_batch_of_reference_encoded_textures(device):
+    """The benchmark's workload at its full size: 128 seeded textures of 512x512 ..
+    2048x2048 (207.9 Mtexel), each file byte for byte the reference encoder's, decoded as
+    one batch with the default shapes (short stream remainders paired) and as a captured
+    graph: every block equals the encoder's."""
+    import os
+    sizes = [(512, 512), (1024, 1024), (512, 1024), (2048, 2048)]
+    picks = np.random.default_rng(42).integers(0, len(sizes), 128)
+    specs = [(1000 + i, sizes[p][0], sizes[p][1]) for i, p in enumerate(picks)]
+    files, dxts = encode.make_textures(specs, nthreads=min(16, os.cpu_count() or 1), encoder="gentenc")
+    blocks, pal = _capacity(files)
+    b = gst_amd.Batch(len(files), blocks, pal)
+    try:
+        hdrs, keep, outs = _bind(device, b, files)
+        assert sum(h.width * h.height for h in hdrs) == 207880192
+        b.decode(device.stream)
+        assert [device.download(o) for o in outs] == dxts
+        b.capture_graph(device.stream)
+        for o in outs:
+            device.poison(o)
+        b.launch_graph(device.stream)
+        assert [device.download(o) for o in outs] == dxts
+        assert b.error_bits(device.stream) == 0
+    finally:
+        b.close()
+
+
 def _garbled(gtc, stream, seed):
     """ref_c0 with 200 bytes of noise in the renormalisation words of one stream (the
     group offsets stay valid): a different bitstream that still decodes everywhere."""
