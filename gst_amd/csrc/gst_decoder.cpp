@@ -214,7 +214,9 @@ class PinnedRing {
     if (slots_.empty()) {
       for (int i = 0; i < kSlots; ++i) {
         Slot sl;
-        GST_HIP(hipHostMalloc(&sl.host, kSlotBytes, hipHostMallocDefault));
+        // Coherent: the device reads a slot itself and must never see a stale cached copy.
+        GST_HIP(hipHostMalloc(&sl.host, kSlotBytes, hipHostMallocCoherent));
+        GST_HIP(hipHostGetDevicePointer(&sl.dev, sl.host, 0));
         GST_HIP(hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming));
         sl.used = false;
         slots_.push_back(sl);
@@ -234,7 +236,12 @@ class PinnedRing {
     next_ = (next_ + 1) % kSlots;
     if (sl.used) GST_HIP(hipEventSynchronize(sl.ev));
     memcpy(sl.host, src, n);
-    GST_HIP(hipMemcpyAsync(d_dst, sl.host, n, hipMemcpyHostToDevice, s));
+    // One gather launch instead of a DMA copy, whose fixed cost (about 10 us) would sit
+    // between the call and its first kernel.
+    gst::GatherArgs ga;
+    ga.n = 1;
+    ga.job[0] = {static_cast<const uint8_t *>(sl.dev), static_cast<uint8_t *>(d_dst), n};
+    GST_HIP(gst::LaunchGather(ga, s));
     GST_HIP(hipEventRecord(sl.ev, s));
     sl.used = true;
     return GST_OK;
@@ -245,6 +252,7 @@ class PinnedRing {
   static constexpr size_t kSlotBytes = 64 << 10;
   struct Slot {
     void *host;
+    void *dev;  // the slot's address for device code
     hipEvent_t ev;
     bool used;
   };
