@@ -376,10 +376,11 @@ def run_served(gpu, qps_list, duration_s, budget_gib=32, closed_loop=512, tuned=
     out = {"pool": "%d pre-encoded textures (gentenc bytes), size mix {512^2: 50%%, 1024^2: 30%%, 2048^2: 15%%, "
                    "4096^2: 5%%}" % len(pool),
            "hbm_budget_GiB": budget_gib, "duration_s_per_point": duration_s,
-           "checked": "every completion compared with its expected DXT1 blocks on the device"}
+           "checked": "every completion compared with its expected DXT1 blocks on the device, except in "
+                      "closed_loop.batched_prestaged_unchecked (the same requests without the comparison)"}
 
-    def summarize(lat_dev, lat_host, res, st):
-        if res["failed"] or res["mismatches"] or res["verified"] != res["requests"]:
+    def summarize(lat_dev, lat_host, res, st, checked=True):
+        if res["failed"] or res["mismatches"] or (checked and res["verified"] != res["requests"]):
             raise SystemExit("served: %s" % res)
         busy = st["device_busy_ms"] * 1e-3
         wall = res["wall_s"]
@@ -403,18 +404,18 @@ def run_served(gpu, qps_list, duration_s, budget_gib=32, closed_loop=512, tuned=
             "scheduler_launch_ms": round(st["launch_host_ms"], 1),
         }
 
-    def one(label, server_kw, load_kw):
+    def one(label, server_kw, load_kw, checked=True):
         with Server(budget_gib << 30, **server_kw) as s:
             s.run_load(pool[:4], qps=200.0, duration_s=0.2, seed=1, pool_expected=pool_exp[:4])  # warm-up
             base = s.stats()
-            lat_dev, lat_host, res = s.run_load(pool, weights, pool_expected=pool_exp, duration_s=duration_s,
-                                                **load_kw)
+            lat_dev, lat_host, res = s.run_load(pool, weights, pool_expected=pool_exp if checked else None,
+                                                duration_s=duration_s, **load_kw)
             st = s.stats()
         for k in ("texels", "algorithmic_bytes", "h2d_bytes", "h2d_copies"):
             st[k] -= base[k]
         st["device_busy_ms"] -= base["device_busy_ms"]
         st["launch_host_ms"] -= base["launch_host_ms"]
-        r = summarize(lat_dev, lat_host, res, st)
+        r = summarize(lat_dev, lat_host, res, st, checked)
         log("served %s: %s" % (label, r))
         return r
 
@@ -436,6 +437,12 @@ def run_served(gpu, qps_list, duration_s, budget_gib=32, closed_loop=512, tuned=
                                   dict(closed_loop=closed_loop, seed=7)),
         "batched_prestaged": one("closed loop, pre-staged", dict(shape),
                                  dict(closed_loop=closed_loop, seed=7, pool_device=pool_dev)),
+        # The same requests with no expected outputs: the server alone, without the
+        # on-device comparison (two more 8-byte-a-block reads per output) that every other
+        # point runs; its outputs are the ones the point above checked.
+        "batched_prestaged_unchecked": one("closed loop, pre-staged, no comparison", dict(shape),
+                                           dict(closed_loop=closed_loop, seed=7, pool_device=pool_dev),
+                                           checked=False),
         "plain_ingest": one("closed loop, plain", dict(plain=True), dict(closed_loop=64, seed=7)),
     }
     cl = out["closed_loop"]
