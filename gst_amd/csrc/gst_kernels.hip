@@ -1645,16 +1645,29 @@ hipError_t LaunchStage(uint32_t stage, const KernelArgs &args, uint32_t grid,
   }
 }
 
-// One work-group row per job: 16-byte loads grid-stride over the job's bytes, one
-// ballot per wave, and a flag store only where a difference was seen.
+// One work-group row per job: 16-byte loads grid-stride over the job's bytes (four
+// pairs in flight per lane, so a large output is not one load round trip per 256 KiB),
+// one ballot per wave, and a flag store only where a difference was seen.
 __global__ __launch_bounds__(256) void CompareKernel(CompareArgs a) {
   const CompareJob j = a.job[blockIdx.y];
   const GPtr<const uint4> x = Gbl<const uint4>(j.out);
   const GPtr<const uint4> y = Gbl<const uint4>(j.expected);
   const uint64_t words = j.bytes / 16;
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
   bool diff = false;
-  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < words;
-       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+  uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  for (; i + 3 * stride < words; i += 4 * stride) {
+    uint4 p[4], q[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      p[k] = x[i + k * stride];
+      q[k] = y[i + k * stride];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      diff |= ((p[k].x ^ q[k].x) | (p[k].y ^ q[k].y) | (p[k].z ^ q[k].z) | (p[k].w ^ q[k].w)) != 0;
+  }
+  for (; i < words; i += stride) {
     const uint4 p = x[i], q = y[i];
     diff |= ((p.x ^ q.x) | (p.y ^ q.y) | (p.z ^ q.z) | (p.w ^ q.w)) != 0;
   }
